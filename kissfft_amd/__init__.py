@@ -1,0 +1,266 @@
+"""kissfft_amd -- Python host binding of the MI355X KISS FFT engine.
+
+The product is the C-ABI library libkissfft-<type>.so (kissfft_amd/lib/), a drop-in for
+the reference's per-datatype libraries: kiss_fft_alloc / kiss_fft / kiss_fft_stride,
+kiss_fftr_alloc / kiss_fftr / kiss_fftri, kiss_fftnd_alloc / kiss_fftnd,
+kiss_fftndr_alloc / kiss_fftndr / kiss_fftndri, kfc_*, plus the batched calls of
+include/kiss_fft_batch.h.  This module binds those symbols with ctypes, the way a caller
+of the reference binds its C API; it holds no compute of its own.
+
+Buffers may be numpy arrays (host memory: the library stages them through HBM) or torch
+tensors on the GPU (device memory: used in place).  Complex data is passed as arrays
+whose last axis is the {re, im} pair of the scalar type (kiss_fft_cpx layout,
+kiss_fft.h:87-90); complex64 numpy/torch arrays have that layout too.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import build as _build
+
+__all__ = ["Library", "load", "SCALARS", "KissFFTError"]
+
+SCALARS = {"float": np.float32, "int16_t": np.int16, "int32_t": np.int32}
+
+
+class KissFFTError(RuntimeError):
+    pass
+
+
+_libc = ctypes.CDLL(None)
+_libc.free.argtypes = [ctypes.c_void_p]
+
+
+def _addr(a) -> int:
+    """Data address of a numpy array or a torch tensor."""
+    if a is None:
+        return 0
+    if isinstance(a, np.ndarray):
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError("buffer must be C-contiguous")
+        return a.ctypes.data
+    if hasattr(a, "data_ptr"):
+        if not a.is_contiguous():
+            raise ValueError("tensor must be contiguous")
+        return a.data_ptr()
+    if isinstance(a, int):
+        return a
+    raise TypeError(f"unsupported buffer type {type(a)}")
+
+
+class Cfg:
+    """An owned cfg pointer (free()'d on collection, as kiss_fft_free does)."""
+
+    def __init__(self, ptr: int, kind: str, **meta):
+        if not ptr:
+            raise KissFFTError(f"{kind} alloc returned NULL for {meta}")
+        self.ptr = ptr
+        self.kind = kind
+        self.meta = meta
+
+    @property
+    def _as_parameter_(self):
+        return ctypes.c_void_p(self.ptr)
+
+    def __del__(self):
+        if getattr(self, "ptr", 0):
+            _libc.free(self.ptr)
+            self.ptr = 0
+
+
+class Library:
+    """ctypes view of one libkissfft-<datatype>.so."""
+
+    def __init__(self, datatype: str = "float"):
+        if datatype not in SCALARS:
+            raise ValueError(f"datatype must be one of {list(SCALARS)}")
+        path = _build.lib_path(datatype)
+        if not os.path.exists(path):
+            raise KissFFTError(f"{path} is not built: run `python -m kissfft_amd.build` "
+                               "(the engine has no CPU fallback)")
+        self.datatype = datatype
+        self.scalar = SCALARS[datatype]
+        self.path = path
+        L = ctypes.CDLL(path)
+        self.L = L
+        vp, ci, ll = ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong
+        szp = ctypes.POINTER(ctypes.c_size_t)
+        ip = ctypes.POINTER(ctypes.c_int)
+        sig = {
+            "kiss_fft_alloc": ([ci, ci, vp, szp], vp),
+            "kiss_fft": ([vp, vp, vp], None),
+            "kiss_fft_stride": ([vp, vp, vp, ci], None),
+            "kiss_fft_cleanup": ([], None),
+            "kiss_fft_next_fast_size": ([ci], ci),
+            "kiss_fftr_alloc": ([ci, ci, vp, szp], vp),
+            "kiss_fftr": ([vp, vp, vp], None),
+            "kiss_fftri": ([vp, vp, vp], None),
+            "kiss_fftnd_alloc": ([ip, ci, ci, vp, szp], vp),
+            "kiss_fftnd": ([vp, vp, vp], None),
+            "kiss_fftndr_alloc": ([ip, ci, ci, vp, szp], vp),
+            "kiss_fftndr": ([vp, vp, vp], None),
+            "kiss_fftndri": ([vp, vp, vp], None),
+            "kfc_fft": ([ci, vp, vp], None),
+            "kfc_ifft": ([ci, vp, vp], None),
+            "kfc_cleanup": ([], None),
+            "kfc_cached_count": ([], ci),
+            "kiss_fft_batch": ([vp, vp, ci, ll, vp, ll, ll], ci),
+            "kiss_fftr_batch": ([vp, vp, ll, vp, ll, ll], ci),
+            "kiss_fftri_batch": ([vp, vp, ll, vp, ll, ll], ci),
+            "kiss_fftnd_batch": ([vp, vp, vp, ll], ci),
+            "kiss_fftndr_batch": ([vp, vp, vp, ll], ci),
+            "kiss_fftndri_batch": ([vp, vp, vp, ll], ci),
+            "kiss_fft_batch_async": ([vp, vp, ci, ll, vp, ll, ll, vp], ci),
+            "kiss_fftr_batch_async": ([vp, vp, ll, vp, ll, ll, vp], ci),
+            "kiss_fftri_batch_async": ([vp, vp, ll, vp, ll, ll, vp], ci),
+            "kiss_fftnd_batch_async": ([vp, vp, vp, ll, vp], ci),
+            "kiss_fftndr_batch_async": ([vp, vp, vp, ll, vp], ci),
+            "kiss_fftndri_batch_async": ([vp, vp, vp, ll, vp], ci),
+            "kiss_fft_stream_sync": ([vp], ci),
+            "kiss_fft_last_error": ([], ctypes.c_char_p),
+            "kiss_fft_device_available": ([], ci),
+            "kiss_fft_get_engine_stats": ([vp], None),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+
+    # -- errors / status --------------------------------------------------------------
+    def last_error(self) -> str:
+        return (self.L.kiss_fft_last_error() or b"").decode()
+
+    def check(self, rc: int, what: str) -> None:
+        if rc != 0:
+            raise KissFFTError(f"{what} failed ({rc}): {self.last_error()}")
+
+    def device_available(self) -> bool:
+        return bool(self.L.kiss_fft_device_available())
+
+    def stats(self) -> dict:
+        buf = (ctypes.c_longlong * 5)()
+        self.L.kiss_fft_get_engine_stats(buf)
+        return dict(zip(["plans_built", "plan_hits", "launches", "h2d_bytes", "d2h_bytes"], list(buf)))
+
+    # -- cfgs -------------------------------------------------------------------------
+    def fft_alloc(self, nfft: int, inverse: bool = False) -> Cfg:
+        return Cfg(self.L.kiss_fft_alloc(nfft, int(inverse), None, None), "kiss_fft", nfft=nfft, inverse=inverse)
+
+    def fftr_alloc(self, nfft: int, inverse: bool = False) -> Cfg:
+        return Cfg(self.L.kiss_fftr_alloc(nfft, int(inverse), None, None), "kiss_fftr", nfft=nfft, inverse=inverse)
+
+    def fftnd_alloc(self, dims, inverse: bool = False) -> Cfg:
+        d = (ctypes.c_int * len(dims))(*dims)
+        return Cfg(self.L.kiss_fftnd_alloc(d, len(dims), int(inverse), None, None), "kiss_fftnd",
+                   dims=tuple(dims), inverse=inverse)
+
+    def fftndr_alloc(self, dims, inverse: bool = False) -> Cfg:
+        d = (ctypes.c_int * len(dims))(*dims)
+        return Cfg(self.L.kiss_fftndr_alloc(d, len(dims), int(inverse), None, None), "kiss_fftndr",
+                   dims=tuple(dims), inverse=inverse)
+
+    def alloc_size(self, kind: str, arg, inverse: bool = False) -> int:
+        """Bytes a cfg needs (the mem=NULL, lenmem!=NULL query)."""
+        n = ctypes.c_size_t(0)
+        if kind in ("kiss_fft", "kiss_fftr"):
+            getattr(self.L, kind + "_alloc")(arg, int(inverse), None, ctypes.byref(n))
+        else:
+            d = (ctypes.c_int * len(arg))(*arg)
+            getattr(self.L, kind + "_alloc")(d, len(arg), int(inverse), None, ctypes.byref(n))
+        return n.value
+
+    # -- reference-shaped single calls (synchronous) -------------------------------------
+    def kiss_fft(self, cfg: Cfg, fin, fout) -> None:
+        self.L.kiss_fft(cfg, _addr(fin), _addr(fout))
+
+    def kiss_fft_stride(self, cfg: Cfg, fin, fout, in_stride: int) -> None:
+        self.L.kiss_fft_stride(cfg, _addr(fin), _addr(fout), in_stride)
+
+    def kiss_fftr(self, cfg: Cfg, timedata, freqdata) -> None:
+        self.L.kiss_fftr(cfg, _addr(timedata), _addr(freqdata))
+
+    def kiss_fftri(self, cfg: Cfg, freqdata, timedata) -> None:
+        self.L.kiss_fftri(cfg, _addr(freqdata), _addr(timedata))
+
+    def kiss_fftnd(self, cfg: Cfg, fin, fout) -> None:
+        self.L.kiss_fftnd(cfg, _addr(fin), _addr(fout))
+
+    def kiss_fftndr(self, cfg: Cfg, timedata, freqdata) -> None:
+        self.L.kiss_fftndr(cfg, _addr(timedata), _addr(freqdata))
+
+    def kiss_fftndri(self, cfg: Cfg, freqdata, timedata) -> None:
+        self.L.kiss_fftndri(cfg, _addr(freqdata), _addr(timedata))
+
+    # -- batched ----------------------------------------------------------------------
+    def fft_batch(self, cfg: Cfg, fin, fout, howmany: int, idist: int, odist: int, in_stride: int = 1,
+                  stream=None) -> None:
+        if stream is None:
+            rc = self.L.kiss_fft_batch(cfg, _addr(fin), in_stride, idist, _addr(fout), odist, howmany)
+        else:
+            rc = self.L.kiss_fft_batch_async(cfg, _addr(fin), in_stride, idist, _addr(fout), odist, howmany,
+                                             _stream_ptr(stream))
+        self.check(rc, "kiss_fft_batch")
+
+    def fftr_batch(self, cfg: Cfg, timedata, freqdata, howmany: int, idist: int, odist: int, stream=None):
+        if stream is None:
+            rc = self.L.kiss_fftr_batch(cfg, _addr(timedata), idist, _addr(freqdata), odist, howmany)
+        else:
+            rc = self.L.kiss_fftr_batch_async(cfg, _addr(timedata), idist, _addr(freqdata), odist, howmany,
+                                              _stream_ptr(stream))
+        self.check(rc, "kiss_fftr_batch")
+
+    def fftri_batch(self, cfg: Cfg, freqdata, timedata, howmany: int, idist: int, odist: int, stream=None):
+        if stream is None:
+            rc = self.L.kiss_fftri_batch(cfg, _addr(freqdata), idist, _addr(timedata), odist, howmany)
+        else:
+            rc = self.L.kiss_fftri_batch_async(cfg, _addr(freqdata), idist, _addr(timedata), odist, howmany,
+                                               _stream_ptr(stream))
+        self.check(rc, "kiss_fftri_batch")
+
+    def fftnd_batch(self, cfg: Cfg, fin, fout, howmany: int, stream=None):
+        if stream is None:
+            rc = self.L.kiss_fftnd_batch(cfg, _addr(fin), _addr(fout), howmany)
+        else:
+            rc = self.L.kiss_fftnd_batch_async(cfg, _addr(fin), _addr(fout), howmany, _stream_ptr(stream))
+        self.check(rc, "kiss_fftnd_batch")
+
+    def fftndr_batch(self, cfg: Cfg, timedata, freqdata, howmany: int, stream=None):
+        if stream is None:
+            rc = self.L.kiss_fftndr_batch(cfg, _addr(timedata), _addr(freqdata), howmany)
+        else:
+            rc = self.L.kiss_fftndr_batch_async(cfg, _addr(timedata), _addr(freqdata), howmany,
+                                                _stream_ptr(stream))
+        self.check(rc, "kiss_fftndr_batch")
+
+    def fftndri_batch(self, cfg: Cfg, freqdata, timedata, howmany: int, stream=None):
+        if stream is None:
+            rc = self.L.kiss_fftndri_batch(cfg, _addr(freqdata), _addr(timedata), howmany)
+        else:
+            rc = self.L.kiss_fftndri_batch_async(cfg, _addr(freqdata), _addr(timedata), howmany,
+                                                 _stream_ptr(stream))
+        self.check(rc, "kiss_fftndri_batch")
+
+    def stream_sync(self, stream=None) -> None:
+        self.check(self.L.kiss_fft_stream_sync(_stream_ptr(stream) if stream is not None else None),
+                   "kiss_fft_stream_sync")
+
+
+def _stream_ptr(stream):
+    """hipStream_t of a torch.cuda.Stream (or a raw integer handle)."""
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+_LIBS: dict[str, Library] = {}
+
+
+def load(datatype: str = "float") -> Library:
+    if datatype not in _LIBS:
+        _LIBS[datatype] = Library(datatype)
+    return _LIBS[datatype]

@@ -1,0 +1,116 @@
+"""Builds the drop-in libraries libkissfft-<type>.so for gfx950, in-tree.
+
+    python -m kissfft_amd.build            # all three datatypes
+    python -m kissfft_amd.build float      # one
+
+Layout (kissfft_amd/lib/):
+    obj/kfa_engine.o                 HIP engine (hipcc --offload-arch=gfx950), one object
+                                     serving every scalar kind
+    libkissfft-<type>.so.131         C host layer of that datatype + the engine; soname
+                                     libkissfft-<type>.so.131, as the reference's
+                                     Makefile:89-113 names its per-datatype libraries
+    libkissfft-<type>.so             symlink to the above
+
+The host layer is compiled with gcc and without -ffast-math (the fixed-point twiddle
+tables depend on exact FP64 rounding).  All HIP code is compiled with packed-FP32
+instructions disabled.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+LIBDIR = os.path.join(PKG, "lib")
+INCLUDE = os.path.join(ROOT, "include")
+
+ARCH = os.environ.get("KISSFFT_AMD_ARCH", "gfx950")
+SOVERSION = "131"
+DATATYPES = {
+    "float": ["-Dkiss_fft_scalar=float"],
+    "int16_t": ["-DFIXED_POINT=16"],
+    "int32_t": ["-DFIXED_POINT=32"],
+}
+HOST_SOURCES = ["kf_core.c", "kf_real.c", "kf_nd.c", "kf_ndr.c", "kf_cache.c"]
+ENGINE_SOURCES = ["kfa_engine.hip"]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CC = os.environ.get("CC", "gcc")
+
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-fPIC",
+    "-std=c++17",
+    "-fvisibility=hidden",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall",
+]
+HOST_FLAGS = ["-O2", "-fPIC", "-std=gnu99", "-fvisibility=hidden", "-DKISS_FFT_SHARED", "-Wall"]
+
+
+def _run(cmd: list[str], verbose: bool) -> None:
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"build step failed ({r.returncode}): {' '.join(cmd[:3])} ...")
+    elif verbose and (r.stderr.strip()):
+        # hipcc's host pass warns that it does not know the packed-fp32 feature: harmless
+        lines = [l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l and l.strip()]
+        if lines:
+            sys.stderr.write("\n".join(lines) + "\n")
+
+
+def lib_path(datatype: str) -> str:
+    return os.path.join(LIBDIR, f"libkissfft-{datatype}.so")
+
+
+def build(datatypes=None, verbose: bool = True) -> list[str]:
+    datatypes = list(datatypes or DATATYPES)
+    objdir = os.path.join(LIBDIR, "obj")
+    os.makedirs(objdir, exist_ok=True)
+    engine_objs = []
+    for src in ENGINE_SOURCES:
+        obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+        _run([HIPCC, *HIP_FLAGS, "-I", CSRC, "-I", INCLUDE, "-c", os.path.join(CSRC, "engine", src), "-o", obj],
+             verbose)
+        engine_objs.append(obj)
+    built = []
+    for dt in datatypes:
+        host_objs = []
+        for src in HOST_SOURCES:
+            obj = os.path.join(objdir, f"{os.path.splitext(src)[0]}-{dt}.o")
+            _run([CC, *HOST_FLAGS, *DATATYPES[dt], "-I", INCLUDE, "-I", CSRC, "-c",
+                  os.path.join(CSRC, "host", src), "-o", obj], verbose)
+            host_objs.append(obj)
+        soname = f"libkissfft-{dt}.so.{SOVERSION}"
+        real = os.path.join(LIBDIR, soname)
+        tmp = real + ".tmp"
+        _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *host_objs, *engine_objs,
+              f"-Wl,-soname,{soname}", "-Wl,-Bsymbolic", "-lpthread", "-lm"], verbose)
+        os.replace(tmp, real)
+        link = lib_path(dt)
+        if os.path.lexists(link):
+            os.remove(link)
+        os.symlink(soname, link)
+        built.append(real)
+    return built
+
+
+def build_oracle(verbose: bool = True) -> None:
+    """Oracle libraries (oracle/lib) and, where the reference tree exists, oracle/_ref."""
+    mk = shutil.which("make") or "make"
+    _run([mk, "-s", "-C", os.path.join(ROOT, "oracle"), "oracle"], verbose)
+    _run([mk, "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], verbose)
+
+
+if __name__ == "__main__":
+    dts = sys.argv[1:] or None
+    for p in build(dts):
+        print("built", p)

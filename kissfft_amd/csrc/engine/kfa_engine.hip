@@ -1,0 +1,823 @@
+// kfa_engine.hip -- device plans, pointer staging and launches behind kfa_engine.h.
+//
+// Device plan = the reference plan moved to HBM: the radix schedule of kf_factor
+// (kiss_fft.c:306-328) executed innermost-first as Stockham stages, and the host-built
+// twiddle table (kiss_fft.c:361-367) uploaded once.  Plans live in a process-wide,
+// mutex-guarded cache keyed by (device, scalar kind, length, direction); the host cfg
+// stays a free()-able block with the reference layout, so it cannot own device memory
+// (kiss_fft.h:136-138) -- kiss_fft_cleanup() releases the cache instead
+// (kiss_fft.c:407-410 is that hook in the reference).
+//
+// Lines up to the LDS capacity run as one pass; longer 1-D lines run as two passes of
+// the same kernel (four-step split of the Stockham schedule at a stage boundary, which
+// keeps every butterfly and twiddle of the reference schedule, so fixed point stays
+// bit-exact).
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../kfa_engine.h"
+#include "kfa_lines.hpp"
+
+namespace kfa {
+
+// ------------------------------------------------------------------------------------
+// errors
+// ------------------------------------------------------------------------------------
+static thread_local char g_err[512];
+
+static int fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define KFA_HIP(call)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(-3, "%s failed: %s", #call, hipGetErrorString(e_));             \
+    } while (0)
+
+static std::atomic<long long> g_plans_built{0}, g_plan_hits{0}, g_launches{0}, g_h2d{0}, g_d2h{0};
+
+static size_t cpx_bytes(int type) { return type == KFA_Q15 ? 4 : 8; }
+
+// largest line kept whole in LDS (two ping-pong buffers must fit 160 KiB)
+static int line_cap(int type) { return type == KFA_Q15 ? 16384 : 8192; }
+
+static const size_t kLdsMax = 160 * 1024;
+
+// ------------------------------------------------------------------------------------
+// device plans
+// ------------------------------------------------------------------------------------
+struct DevPlan {
+    int type = 0, n = 0, inverse = 0;
+    std::vector<int> radix;  // execution order (innermost factor first)
+    void *d_tw = nullptr;    // n complex
+    bool single = true;
+    LinePlan lp{};           // single pass
+    int L1 = 0, Q = 0;       // four-step: L1 * Q == n
+    LinePlan lp1{}, lp2{};
+};
+
+struct RealTables {
+    void *d_stw = nullptr;  // nfft/4 complex
+};
+
+struct Cache {
+    std::mutex mu;
+    std::map<std::tuple<int, int, int, int>, std::shared_ptr<DevPlan>> cplans;   // dev,type,n,inv
+    std::map<std::tuple<int, int, int, int>, std::shared_ptr<RealTables>> rtabs; // dev,type,nreal,inv
+};
+
+static Cache &cache()
+{
+    static Cache *c = new Cache();  // never destroyed: device memory is released by kfa_cleanup
+    return *c;
+}
+
+static void fill_stages(LinePlan &lp, const std::vector<int> &radix, int s0, int s1, int n, int L,
+                        int div_P, int div_m, int u_mul, int u_add_lo, int inverse)
+{
+    // global sub-length m before stage s is the product of the radices executed before it
+    long long mg = 1;
+    for (int s = 0; s < s0; ++s)
+        mg *= radix[s];
+    lp.L = L;
+    lp.N = n;
+    lp.nst = s1 - s0;
+    lp.inverse = inverse;
+    lp.u_mul = u_mul;
+    lp.u_add_lo = u_add_lo;
+    for (int s = s0; s < s1; ++s) {
+        const int p = radix[s];
+        const long long Pg = n / (p * mg);
+        StageDesc &d = lp.st[s - s0];
+        d.p = p;
+        d.mg = (int)mg;
+        d.Ptw = (int)Pg;
+        d.ml = (int)(mg / div_m);
+        d.Pl = (int)(Pg / div_P);
+        mg *= p;
+    }
+}
+
+static int build_cplan(const kfa_cplan *hp, int dev, std::shared_ptr<DevPlan> &out)
+{
+    auto &C = cache();
+    const auto key = std::make_tuple(dev, hp->type, hp->nfft, hp->inverse ? 1 : 0);
+    {
+        std::lock_guard<std::mutex> g(C.mu);
+        auto it = C.cplans.find(key);
+        if (it != C.cplans.end()) {
+            out = it->second;
+            ++g_plan_hits;
+            return 0;
+        }
+    }
+    auto P = std::make_shared<DevPlan>();
+    P->type = hp->type;
+    P->n = hp->nfft;
+    P->inverse = hp->inverse ? 1 : 0;
+    // factors are (p, m) pairs from the outermost level; execution is innermost first
+    std::vector<int> outer;
+    for (int k = 0; k < 64; ++k) {
+        outer.push_back(hp->factors[2 * k]);
+        if (hp->factors[2 * k + 1] <= 1)
+            break;
+    }
+    P->radix.assign(outer.rbegin(), outer.rend());
+    const int K = (int)P->radix.size();
+    const int n = P->n;
+    const int cap = line_cap(hp->type);
+    if (n <= cap) {
+        P->single = true;
+        fill_stages(P->lp, P->radix, 0, K, n, n, 1, 1, 1, 0, P->inverse);
+    } else {
+        int best = -1;
+        long long bestmax = 0;
+        long long L1 = 1;
+        for (int j = 1; j < K; ++j) {
+            L1 *= P->radix[j - 1];
+            const long long Q = n / L1;
+            if (L1 <= cap && Q <= cap) {
+                const long long mx = L1 > Q ? L1 : Q;
+                if (best < 0 || mx < bestmax) {
+                    best = j;
+                    bestmax = mx;
+                }
+            }
+        }
+        if (best < 0)
+            return fail(-2, "nfft=%d: a radix factor exceeds the %d-point LDS line; not supported", n, cap);
+        long long l1 = 1;
+        for (int j = 0; j < best; ++j)
+            l1 *= P->radix[j];
+        P->single = false;
+        P->L1 = (int)l1;
+        P->Q = (int)(n / l1);
+        // pass 1: lines are the Q decimated columns, local fstride P/Q
+        fill_stages(P->lp1, P->radix, 0, best, n, P->L1, P->Q, 1, 1, 0, P->inverse);
+        // pass 2: lines are the L1 rows of length Q, column index u = lo + L1*u_local
+        fill_stages(P->lp2, P->radix, best, K, n, P->Q, 1, P->L1, P->L1, 1, P->inverse);
+    }
+    const size_t tw_bytes = cpx_bytes(hp->type) * (size_t)n;
+    KFA_HIP(hipMalloc(&P->d_tw, tw_bytes));
+    KFA_HIP(hipMemcpy(P->d_tw, hp->twiddles, tw_bytes, hipMemcpyHostToDevice));
+    ++g_plans_built;
+    std::lock_guard<std::mutex> g(C.mu);
+    auto it = C.cplans.find(key);
+    if (it != C.cplans.end()) {  // another thread won the race
+        (void)hipFree(P->d_tw);
+        P->d_tw = nullptr;
+        out = it->second;
+        return 0;
+    }
+    C.cplans[key] = P;
+    out = P;
+    return 0;
+}
+
+static int build_rtab(const kfa_rplan *hp, int dev, std::shared_ptr<RealTables> &out)
+{
+    auto &C = cache();
+    const int nreal = 2 * hp->half.nfft;
+    const auto key = std::make_tuple(dev, hp->half.type, nreal, hp->half.inverse ? 1 : 0);
+    {
+        std::lock_guard<std::mutex> g(C.mu);
+        auto it = C.rtabs.find(key);
+        if (it != C.rtabs.end()) {
+            out = it->second;
+            return 0;
+        }
+    }
+    auto R = std::make_shared<RealTables>();
+    const size_t cnt = (size_t)(hp->half.nfft / 2);
+    const size_t bytes = cpx_bytes(hp->half.type) * (cnt > 0 ? cnt : 1);
+    KFA_HIP(hipMalloc(&R->d_stw, bytes));
+    if (cnt)
+        KFA_HIP(hipMemcpy(R->d_stw, hp->super_twiddles, bytes, hipMemcpyHostToDevice));
+    std::lock_guard<std::mutex> g(C.mu);
+    auto it = C.rtabs.find(key);
+    if (it != C.rtabs.end()) {
+        (void)hipFree(R->d_stw);
+        out = it->second;
+        return 0;
+    }
+    C.rtabs[key] = R;
+    out = R;
+    return 0;
+}
+
+// ------------------------------------------------------------------------------------
+// streams, pointers, staging
+// ------------------------------------------------------------------------------------
+static thread_local hipStream_t t_stream = nullptr;
+
+static int thread_stream(hipStream_t *s)
+{
+    if (!t_stream)
+        KFA_HIP(hipStreamCreateWithFlags(&t_stream, hipStreamNonBlocking));
+    *s = t_stream;
+    return 0;
+}
+
+static bool device_accessible(const void *p)
+{
+    if (!p)
+        return false;
+    hipPointerAttribute_t a;
+    hipError_t e = hipPointerGetAttributes(&a, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged ||
+           a.type == hipMemoryTypeUnified;
+}
+
+static int current_device(int *dev)
+{
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0) {
+        (void)hipGetLastError();
+        return fail(-1, "no HIP device available: the MI355X engine cannot run (no CPU fallback)");
+    }
+    KFA_HIP(hipGetDevice(dev));
+    return 0;
+}
+
+// A call's view of one operand: the caller's pointer, or a stream-ordered HBM copy.
+struct Operand {
+    const void *host = nullptr;  // non-null when staged
+    void *dev = nullptr;
+    size_t bytes = 0;
+    bool staged = false;
+};
+
+static int stage_in(const void *p, size_t bytes, hipStream_t s, Operand &op)
+{
+    op.bytes = bytes;
+    if (device_accessible(p)) {
+        op.dev = const_cast<void *>(p);
+        return 0;
+    }
+    op.staged = true;
+    op.host = p;
+    KFA_HIP(hipMallocAsync(&op.dev, bytes ? bytes : 1, s));
+    if (bytes) {
+        KFA_HIP(hipMemcpyAsync(op.dev, p, bytes, hipMemcpyHostToDevice, s));
+        g_h2d += (long long)bytes;
+    }
+    return 0;
+}
+
+static int stage_out(void *p, size_t bytes, hipStream_t s, Operand &op)
+{
+    op.bytes = bytes;
+    if (device_accessible(p)) {
+        op.dev = p;
+        return 0;
+    }
+    op.staged = true;
+    op.host = p;
+    KFA_HIP(hipMallocAsync(&op.dev, bytes ? bytes : 1, s));
+    return 0;
+}
+
+static int finish_out(Operand &op, hipStream_t s)
+{
+    if (op.staged && op.bytes) {
+        KFA_HIP(hipMemcpyAsync(const_cast<void *>(op.host), op.dev, op.bytes, hipMemcpyDeviceToHost, s));
+        g_d2h += (long long)op.bytes;
+    }
+    return 0;
+}
+
+static void release(Operand &op, hipStream_t s)
+{
+    if (op.staged && op.dev)
+        (void)hipFreeAsync(op.dev, s);
+    op.dev = nullptr;
+}
+
+// ------------------------------------------------------------------------------------
+// launches of the line kernel
+// ------------------------------------------------------------------------------------
+struct LineLaunch {
+    const void *in;
+    void *out;
+    const void *tw;
+    const void *stw;
+    const LinePlan *pl;
+    LineIO io;
+    int colmode;
+    int prologue;
+    int epilogue;
+};
+
+template <class A>
+static int launch_lines_t(const LineLaunch &ll, hipStream_t s)
+{
+    typedef typename A::store2 S;
+    const int L = ll.pl->L;
+    const size_t cb = sizeof(S);
+    const size_t per_line = 2 * (size_t)L * cb;
+    if (per_line > kLdsMax)
+        return fail(-2, "line of %d points exceeds LDS", L);
+    int tl;
+    if (ll.colmode) {
+        tl = L >= 256 ? 16 : 64;
+    } else {
+        tl = 2048 / L;
+        if (tl < 1)
+            tl = 1;
+        if (tl > 64)
+            tl = 64;
+    }
+    while (tl > 1 && per_line * tl > 64 * 1024)
+        tl >>= 1;
+    if ((long long)tl > ll.io.nlines)
+        tl = (int)ll.io.nlines;
+    if (tl < 1)
+        tl = 1;
+    const size_t lds = per_line * tl;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        (void)hipFuncSetAttribute((const void *)k_lines<A>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)kLdsMax);
+    });
+    const long long blocks = (ll.io.nlines + tl - 1) / tl;
+    if (blocks <= 0)
+        return 0;
+    if (blocks > 0x7fffffffLL)
+        return fail(-2, "too many lines (%lld) for one launch", ll.io.nlines);
+    hipLaunchKernelGGL(k_lines<A>, dim3((unsigned)blocks), dim3(256), lds, s, (const S *)ll.in, (S *)ll.out,
+                       (const S *)ll.tw, (const S *)ll.stw, *ll.pl, ll.io, tl, ll.colmode, ll.prologue,
+                       ll.epilogue);
+    KFA_HIP(hipGetLastError());
+    ++g_launches;
+    return 0;
+}
+
+static int launch_lines(int type, const LineLaunch &ll, hipStream_t s)
+{
+    switch (type) {
+    case KFA_F32: return launch_lines_t<F32>(ll, s);
+    case KFA_Q15: return launch_lines_t<Q15>(ll, s);
+    case KFA_Q31: return launch_lines_t<Q31>(ll, s);
+    default: return fail(-2, "unknown scalar kind %d", type);
+    }
+}
+
+// Runs `howmany` complex transforms of plan P between two device buffers.
+// in: element t of item b at in + (b*idist + t*istride); out: out + (b*odist + t*ostride).
+static int run_c2c(const DevPlan &P, const void *in, long long istride, long long idist, void *out,
+                   long long ostride, long long odist, long long howmany, hipStream_t s)
+{
+    const int n = P.n;
+    if (howmany <= 0 || n <= 0)
+        return 0;
+    if (P.single) {
+        LineLaunch ll{};
+        ll.in = in;
+        ll.out = out;
+        ll.tw = P.d_tw;
+        ll.pl = &P.lp;
+        ll.io.nlines = howmany;
+        ll.io.nlo = 1;
+        ll.io.in_hi = idist;
+        ll.io.in_lo = 0;
+        ll.io.in_el = istride;
+        ll.io.out_hi = odist;
+        ll.io.out_lo = 0;
+        ll.io.out_el = ostride;
+        ll.colmode = 0;
+        return launch_lines(P.type, ll, s);
+    }
+    // four-step: pass 1 into a stream-ordered temporary, pass 2 into out
+    void *tmp = nullptr;
+    const size_t tbytes = cpx_bytes(P.type) * (size_t)n * (size_t)howmany;
+    KFA_HIP(hipMallocAsync(&tmp, tbytes, s));
+    LineLaunch a{};
+    a.in = in;
+    a.out = tmp;
+    a.tw = P.d_tw;
+    a.pl = &P.lp1;
+    a.io.nlines = howmany * P.Q;
+    a.io.nlo = P.Q;
+    a.io.in_hi = idist;
+    a.io.in_lo = istride;
+    a.io.in_el = (long long)P.Q * istride;
+    a.io.out_hi = n;
+    a.io.out_lo = 1;
+    a.io.out_el = P.Q;
+    a.colmode = 1;
+    int rc = launch_lines(P.type, a, s);
+    if (rc == 0) {
+        LineLaunch b{};
+        b.in = tmp;
+        b.out = out;
+        b.tw = P.d_tw;
+        b.pl = &P.lp2;
+        b.io.nlines = howmany * P.L1;
+        b.io.nlo = P.L1;
+        b.io.in_hi = n;
+        b.io.in_lo = P.Q;
+        b.io.in_el = 1;
+        b.io.out_hi = odist;
+        b.io.out_lo = ostride;
+        b.io.out_el = (long long)P.L1 * ostride;
+        b.colmode = 1;
+        rc = launch_lines(P.type, b, s);
+    }
+    (void)hipFreeAsync(tmp, s);
+    return rc;
+}
+
+static int begin(hipStream_t *s, void *stream, int *dev)
+{
+    int rc = current_device(dev);
+    if (rc)
+        return rc;
+    if (stream) {
+        *s = (hipStream_t)stream;
+        return 0;
+    }
+    return thread_stream(s);
+}
+
+static int end_call(hipStream_t s, int sync, int rc)
+{
+    if (sync || rc) {
+        hipError_t e = hipStreamSynchronize(s);
+        if (rc == 0 && e != hipSuccess)
+            return fail(-3, "stream synchronize: %s", hipGetErrorString(e));
+    }
+    return rc;
+}
+
+}  // namespace kfa
+
+using namespace kfa;
+
+// ------------------------------------------------------------------------------------
+// C ABI
+// ------------------------------------------------------------------------------------
+extern "C" {
+
+const char *kfa_last_error(void) { return g_err; }
+
+int kfa_device_available(void)
+{
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return count > 0;
+}
+
+void kfa_get_stats(kfa_stats *st)
+{
+    st->plans_built = g_plans_built.load();
+    st->plan_hits = g_plan_hits.load();
+    st->launches = g_launches.load();
+    st->h2d_bytes = g_h2d.load();
+    st->d2h_bytes = g_d2h.load();
+}
+
+int kfa_c2c(const kfa_cplan *pl, const void *in, long long istride, long long idist, void *out,
+            long long odist, long long howmany, void *stream, int sync)
+{
+    hipStream_t s;
+    int dev;
+    int rc = begin(&s, stream, &dev);
+    if (rc)
+        return rc;
+    std::shared_ptr<DevPlan> P;
+    if ((rc = build_cplan(pl, dev, P)))
+        return rc;
+    const size_t cb = cpx_bytes(pl->type);
+    const long long n = pl->nfft;
+    if (howmany <= 0)
+        return 0;
+    const size_t in_bytes = cb * (size_t)((howmany - 1) * idist + (n - 1) * istride + 1);
+    const size_t out_bytes = cb * (size_t)((howmany - 1) * odist + n);
+    Operand I, O;
+    if ((rc = stage_in(in, in_bytes, s, I)) == 0 && (rc = stage_out(out, out_bytes, s, O)) == 0) {
+        const bool same = I.dev == O.dev && !I.staged && !O.staged;
+        const bool inplace_ok = !same || (P->single && istride == 1 && idist == odist);
+        if (inplace_ok) {
+            rc = run_c2c(*P, I.dev, istride, idist, O.dev, 1, odist, howmany, s);
+        } else {
+            // device in == out with a layout the single pass cannot overwrite in place
+            void *tmp = nullptr;
+            KFA_HIP(hipMallocAsync(&tmp, out_bytes, s));
+            rc = run_c2c(*P, I.dev, istride, idist, tmp, 1, odist, howmany, s);
+            if (rc == 0)
+                KFA_HIP(hipMemcpyAsync(O.dev, tmp, out_bytes, hipMemcpyDeviceToDevice, s));
+            (void)hipFreeAsync(tmp, s);
+        }
+        if (rc == 0)
+            rc = finish_out(O, s);
+    }
+    release(I, s);
+    release(O, s);
+    return end_call(s, sync || I.staged || O.staged, rc);
+}
+
+static int real_common(const kfa_rplan *pl, const void *in, long long idist, void *out, long long odist,
+                       long long howmany, void *stream, int sync, int inverse)
+{
+    hipStream_t s;
+    int dev;
+    int rc = begin(&s, stream, &dev);
+    if (rc)
+        return rc;
+    std::shared_ptr<DevPlan> P;
+    std::shared_ptr<RealTables> R;
+    if ((rc = build_cplan(&pl->half, dev, P)))
+        return rc;
+    if ((rc = build_rtab(pl, dev, R)))
+        return rc;
+    if (howmany <= 0)
+        return 0;
+    const long long nc = pl->half.nfft;
+    const size_t sb = cpx_bytes(pl->half.type) / 2;  // real scalar bytes
+    const size_t cb = 2 * sb;
+    // real side counts scalars, complex side counts complex values
+    const size_t real_bytes_in = sb * (size_t)((howmany - 1) * idist + 2 * nc);
+    const size_t cpx_bytes_in = cb * (size_t)((howmany - 1) * idist + nc + 1);
+    const size_t real_bytes_out = sb * (size_t)((howmany - 1) * odist + 2 * nc);
+    const size_t cpx_bytes_out = cb * (size_t)((howmany - 1) * odist + nc + 1);
+    Operand I, O;
+    rc = stage_in(in, inverse ? cpx_bytes_in : real_bytes_in, s, I);
+    if (rc == 0)
+        rc = stage_out(out, inverse ? real_bytes_out : cpx_bytes_out, s, O);
+    if (rc == 0) {
+        if (!P->single) {
+            rc = fail(-2, "real transform of %lld points: half length exceeds one LDS line (not yet supported)",
+                      2 * nc);
+        } else {
+            LineLaunch ll{};
+            ll.tw = P->d_tw;
+            ll.stw = R->d_stw;
+            ll.pl = &P->lp;
+            ll.io.nlines = howmany;
+            ll.io.nlo = 1;
+            ll.colmode = 0;
+            if (!inverse) {
+                if (idist & 1) {
+                    rc = fail(-2, "real input distance must be even (complex-pair loads)");
+                } else {
+                    ll.in = I.dev;
+                    ll.out = O.dev;
+                    ll.io.in_hi = idist / 2;  // in complex pairs
+                    ll.io.in_el = 1;
+                    ll.io.out_hi = odist;
+                    ll.io.out_el = 1;
+                    ll.epilogue = EPI_R2C;
+                    rc = launch_lines(pl->half.type, ll, s);
+                }
+            } else {
+                if (odist & 1) {
+                    rc = fail(-2, "real output distance must be even (complex-pair stores)");
+                } else {
+                    ll.in = I.dev;
+                    ll.out = O.dev;
+                    ll.io.in_hi = idist;
+                    ll.io.in_el = 1;
+                    ll.io.out_hi = odist / 2;
+                    ll.io.out_el = 1;
+                    ll.prologue = PRO_C2R;
+                    rc = launch_lines(pl->half.type, ll, s);
+                }
+            }
+        }
+        if (rc == 0)
+            rc = finish_out(O, s);
+    }
+    release(I, s);
+    release(O, s);
+    return end_call(s, sync || I.staged || O.staged, rc);
+}
+
+int kfa_r2c(const kfa_rplan *pl, const void *in, long long idist, void *out, long long odist,
+            long long howmany, void *stream, int sync)
+{
+    return real_common(pl, in, idist, out, odist, howmany, stream, sync, 0);
+}
+
+int kfa_c2r(const kfa_rplan *pl, const void *in, long long idist, void *out, long long odist,
+            long long howmany, void *stream, int sync)
+{
+    return real_common(pl, in, idist, out, odist, howmany, stream, sync, 1);
+}
+
+// One axis of an N-D array batch, dims[k] long, between device buffers (in may equal out).
+static int run_axis(const DevPlan &P, const void *in, void *out, long long outer, long long inner,
+                    hipStream_t s)
+{
+    const long long len = P.n;
+    if (!P.single) {
+        // long axis: run the two-pass transform per outer block and inner column
+        if (inner != 1)
+            return fail(-2, "N-D axis of %lld points with inner stride %lld: not yet supported", len, inner);
+        return run_c2c(P, in, 1, len, out, 1, len, outer, s);
+    }
+    LineLaunch ll{};
+    ll.in = in;
+    ll.out = out;
+    ll.tw = P.d_tw;
+    ll.pl = &P.lp;
+    ll.io.nlines = outer * inner;
+    ll.io.nlo = inner;
+    ll.io.in_hi = len * inner;
+    ll.io.in_lo = 1;
+    ll.io.in_el = inner;
+    ll.io.out_hi = len * inner;
+    ll.io.out_lo = 1;
+    ll.io.out_el = inner;
+    ll.colmode = inner > 1 ? 1 : 0;
+    return launch_lines(P.type, ll, s);
+}
+
+int kfa_nd(int ndims, const int *dims, const kfa_cplan *axes, const void *in, void *out,
+           long long howmany, void *stream, int sync)
+{
+    hipStream_t s;
+    int dev;
+    int rc = begin(&s, stream, &dev);
+    if (rc)
+        return rc;
+    if (ndims <= 0 || howmany <= 0)
+        return 0;
+    std::vector<std::shared_ptr<DevPlan>> P(ndims);
+    long long total = 1;
+    for (int k = 0; k < ndims; ++k) {
+        if ((rc = build_cplan(&axes[k], dev, P[k])))
+            return rc;
+        total *= dims[k];
+    }
+    const size_t bytes = cpx_bytes(axes[0].type) * (size_t)(total * howmany);
+    Operand I, O;
+    rc = stage_in(in, bytes, s, I);
+    if (rc == 0)
+        rc = stage_out(out, bytes, s, O);
+    if (rc == 0) {
+        // axis 0 first, as kiss_fftnd.c:181-186 does; fixed point rounds per axis, so the
+        // order is part of the result.  The first axis reads the input, later axes run in
+        // place on the output.
+        long long outer = howmany;
+        long long inner = total;
+        const void *src = I.dev;
+        for (int k = 0; k < ndims && rc == 0; ++k) {
+            inner /= dims[k];
+            rc = run_axis(*P[k], src, O.dev, outer, inner, s);
+            outer *= dims[k];
+            src = O.dev;
+        }
+        if (rc == 0)
+            rc = finish_out(O, s);
+    }
+    release(I, s);
+    release(O, s);
+    return end_call(s, sync || I.staged || O.staged, rc);
+}
+
+int kfa_ndr(int ndims, const int *dims, const kfa_rplan *rpl, const kfa_cplan *axes, int inverse,
+            const void *in, void *out, long long howmany, void *stream, int sync)
+{
+    hipStream_t s;
+    int dev;
+    int rc = begin(&s, stream, &dev);
+    if (rc)
+        return rc;
+    if (ndims <= 0 || howmany <= 0)
+        return 0;
+    std::shared_ptr<DevPlan> H;
+    std::shared_ptr<RealTables> R;
+    if ((rc = build_cplan(&rpl->half, dev, H)) || (rc = build_rtab(rpl, dev, R)))
+        return rc;
+    std::vector<std::shared_ptr<DevPlan>> P(ndims > 1 ? ndims - 1 : 0);
+    long long rows = 1;
+    for (int k = 0; k + 1 < ndims; ++k) {
+        if ((rc = build_cplan(&axes[k], dev, P[k])))
+            return rc;
+        rows *= dims[k];
+    }
+    if (!H->single)
+        return fail(-2, "real axis of %d points exceeds one LDS line (not yet supported)", dims[ndims - 1]);
+    const long long nreal = dims[ndims - 1];
+    const long long nbins = nreal / 2 + 1;
+    const size_t sb = cpx_bytes(rpl->half.type) / 2;
+    const size_t real_bytes = sb * (size_t)(rows * nreal * howmany);
+    const size_t cpx_bytes_ = 2 * sb * (size_t)(rows * nbins * howmany);
+    Operand I, O;
+    rc = stage_in(in, inverse ? cpx_bytes_ : real_bytes, s, I);
+    if (rc == 0)
+        rc = stage_out(out, inverse ? real_bytes : cpx_bytes_, s, O);
+    if (rc == 0) {
+        LineLaunch ll{};
+        ll.tw = H->d_tw;
+        ll.stw = R->d_stw;
+        ll.pl = &H->lp;
+        ll.io.nlines = rows * howmany;
+        ll.io.nlo = 1;
+        if (!inverse) {
+            // kiss_fftndr.c:99-109: real rows first, then the complex axes per bin column
+            ll.in = I.dev;
+            ll.out = O.dev;
+            ll.io.in_hi = nreal / 2;
+            ll.io.in_el = 1;
+            ll.io.out_hi = nbins;
+            ll.io.out_el = 1;
+            ll.epilogue = EPI_R2C;
+            rc = launch_lines(rpl->half.type, ll, s);
+            long long outer = howmany, inner = rows * nbins;
+            for (int k = 0; k + 1 < ndims && rc == 0; ++k) {
+                inner /= dims[k];
+                rc = run_axis(*P[k], O.dev, O.dev, outer, inner, s);
+                outer *= dims[k];
+            }
+        } else {
+            // kiss_fftndr.c:121-131: complex axes per bin column first, then real rows.
+            // The input is left untouched: the axes run out of place into a temporary.
+            void *tmp = nullptr;
+            const size_t tb = cpx_bytes_;
+            KFA_HIP(hipMallocAsync(&tmp, tb, s));
+            long long outer = howmany, inner = rows * nbins;
+            const void *src = I.dev;
+            for (int k = 0; k + 1 < ndims && rc == 0; ++k) {
+                inner /= dims[k];
+                rc = run_axis(*P[k], src, tmp, outer, inner, s);
+                outer *= dims[k];
+                src = tmp;
+            }
+            if (ndims == 1)
+                KFA_HIP(hipMemcpyAsync(tmp, I.dev, tb, hipMemcpyDeviceToDevice, s));
+            if (rc == 0) {
+                ll.in = tmp;
+                ll.out = O.dev;
+                ll.io.in_hi = nbins;
+                ll.io.in_el = 1;
+                ll.io.out_hi = nreal / 2;
+                ll.io.out_el = 1;
+                ll.prologue = PRO_C2R;
+                rc = launch_lines(rpl->half.type, ll, s);
+            }
+            (void)hipFreeAsync(tmp, s);
+        }
+        if (rc == 0)
+            rc = finish_out(O, s);
+    }
+    release(I, s);
+    release(O, s);
+    return end_call(s, sync || I.staged || O.staged, rc);
+}
+
+int kfa_stream_sync(void *stream)
+{
+    hipStream_t s;
+    int dev;
+    int rc = begin(&s, stream, &dev);
+    if (rc)
+        return rc;
+    KFA_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+void kfa_cleanup(void)
+{
+    auto &C = cache();
+    std::lock_guard<std::mutex> g(C.mu);
+    for (auto &kv : C.cplans)
+        if (kv.second->d_tw) {
+            (void)hipFree(kv.second->d_tw);
+            kv.second->d_tw = nullptr;
+        }
+    for (auto &kv : C.rtabs)
+        if (kv.second->d_stw) {
+            (void)hipFree(kv.second->d_stw);
+            kv.second->d_stw = nullptr;
+        }
+    C.cplans.clear();
+    C.rtabs.clear();
+}
+
+}  // extern "C"

@@ -1,0 +1,274 @@
+// kfa_lines.hpp -- the general LDS-resident Stockham "line" kernel.
+//
+// A launch transforms `nlines` independent lines.  Line l = (hi, lo) with lo = l % nlo
+// reads element t at  in[hi*in_hi + lo*in_lo + t*in_el]  and writes element t at
+// out[hi*out_hi + lo*out_lo + t*out_el].  With suitable strides one kernel serves a batch
+// of 1-D transforms (kiss_fft_stride's in_stride included), the per-axis passes of the
+// N-D transform (kiss_fftnd.c:172-177 walks the same columns), and both passes of the
+// four-step split used for lines longer than LDS holds.
+//
+// A workgroup owns a tile of TL consecutive lines.  The tile is read once from HBM into
+// LDS, every radix stage runs LDS -> LDS (ping-pong), and the result is written once.
+// LDS layout: element (line, t) sits at t*TL + line in "column" tiles (lines adjacent in
+// HBM, so a wave reads TL neighbours per row) and at line*L + t in "row" tiles.
+//
+// Stage arithmetic is the Stockham form of the reference's decimation in time: with
+// radix p, sub-length m and fstride P = L/(p*m), butterfly b = u*P + o reads inputs
+// u*P*p + r*P + o (r < p) and writes outputs b + q*(L/p) (q < p), using twiddles
+// table[r*u*P] -- exactly the values kf_bfly* (kiss_fft.c:15-233) combines at column u of
+// the recursion, so fixed-point results are bit-identical.  For the second pass of a
+// four-step split the stage's column index is u_global = lo + u_mul*u and the twiddle
+// index uses the global fstride Ptw.
+#pragma once
+
+#include "kfa_arith.hpp"
+
+#define KFA_MAX_STAGES 32
+
+namespace kfa {
+
+struct StageDesc {
+    int p;      // radix
+    int ml;     // local sub-length m (in line elements)
+    int Pl;     // local fstride = L / (p * ml)
+    int Ptw;    // fstride in the full-length twiddle table
+    int mg;     // global sub-length m (table index of ya is Ptw*mg)
+};
+
+struct LinePlan {
+    int L;        // line length
+    int N;        // full transform length (twiddle table length)
+    int nst;      // number of stages
+    int inverse;
+    int u_mul;    // u_global = u_add + u_mul * u_local
+    int u_add_lo; // 1: u_add is the line's lo index (four-step second pass), else 0
+    StageDesc st[KFA_MAX_STAGES];
+};
+
+struct LineIO {
+    long long nlines;
+    long long nlo;
+    long long in_hi, in_lo, in_el;
+    long long out_hi, out_lo, out_el;
+};
+
+enum { EPI_NONE = 0, EPI_R2C = 1 };
+enum { PRO_NONE = 0, PRO_C2R = 1 };
+
+template <class A>
+struct LineCtx {
+    typedef typename A::store2 S;
+    int TL, nl, L, colmode;
+    __device__ __forceinline__ int idx(int line, int t) const { return colmode ? t * TL + line : line * L + t; }
+    // flat work item -> (line, j) for j in [0, n)
+    __device__ __forceinline__ void split(int w, int n, int &line, int &j) const
+    {
+        if (colmode) {
+            line = w % TL;
+            j = w / TL;
+        } else {
+            line = w / n;
+            j = w - line * n;
+        }
+    }
+};
+
+template <class A>
+__device__ __forceinline__ void run_stage(const LinePlan &pl, const StageDesc &sd, const LineCtx<A> &cx,
+                                          const typename A::store2 *src, typename A::store2 *dst,
+                                          const typename A::store2 *__restrict__ tw, long long l0,
+                                          long long nlo)
+{
+    typedef Ops<A> O;
+    typedef Cx<A> C;
+    const int p = sd.p, L = cx.L, nb = L / p, Pl = sd.Pl;
+    const int nthreads = blockDim.x;
+    if (p <= 5) {
+        const C ya = O::load(tw[(long long)sd.Ptw * sd.mg % pl.N]);
+        const C yb = O::load(tw[2LL * sd.Ptw * sd.mg % pl.N]);
+        for (int w = threadIdx.x; w < cx.TL * nb; w += nthreads) {
+            int line, b;
+            cx.split(w, nb, line, b);
+            if (line >= cx.nl)
+                continue;
+            const int u = b / Pl, o = b - (b / Pl) * Pl;
+            const long long uadd = pl.u_add_lo ? (l0 + line) % nlo : 0;
+            const long long ug = uadd + (long long)pl.u_mul * u;
+            C v[5], wv[4];
+            for (int r = 0; r < p; ++r)
+                v[r] = O::load(src[cx.idx(line, u * Pl * p + r * Pl + o)]);
+            for (int r = 1; r < p; ++r)
+                wv[r - 1] = O::load(tw[(long long)r * ug * sd.Ptw]);
+            switch (p) {
+            case 2: bfly2<A>(v, wv); break;
+            case 3: bfly3<A>(v, wv, ya); break;
+            case 4: bfly4<A>(v, wv, pl.inverse != 0); break;
+            default: bfly5<A>(v, wv, ya, yb); break;
+            }
+            for (int q = 0; q < p; ++q)
+                dst[cx.idx(line, b + q * nb)] = O::pack(v[q]);
+        }
+    } else {
+        // Generic radix (kiss_fft.c:191-233): one work item per output k = u + q1*m.
+        for (int w = threadIdx.x; w < cx.TL * L; w += nthreads) {
+            int line, j;
+            cx.split(w, L, line, j);
+            if (line >= cx.nl)
+                continue;
+            const int q1 = j / nb, b = j - (j / nb) * nb;
+            const int u = b / Pl, o = b - (b / Pl) * Pl;
+            const long long uadd = pl.u_add_lo ? (l0 + line) % nlo : 0;
+            const long long ug = uadd + (long long)pl.u_mul * u;
+            const long long kg = ug + (long long)q1 * sd.mg;
+            const long long step = (long long)sd.Ptw * kg % pl.N;
+            C acc = O::fixdiv(O::load(src[cx.idx(line, u * Pl * p + o)]), p);
+            long long ti = 0;
+            for (int q = 1; q < p; ++q) {
+                ti += step;
+                if (ti >= pl.N)
+                    ti -= pl.N;
+                C x = O::fixdiv(O::load(src[cx.idx(line, u * Pl * p + q * Pl + o)]), p);
+                acc = O::cadd(acc, O::cmul(x, O::load(tw[ti])));
+            }
+            dst[cx.idx(line, j)] = O::pack(acc);
+        }
+    }
+}
+
+// Real-input split after the half-length transform, kiss_fftr.c:88-116.
+template <class A>
+__device__ __forceinline__ void r2c_epilogue(const LineCtx<A> &cx, const typename A::store2 *z,
+                                             typename A::store2 *__restrict__ out,
+                                             const typename A::store2 *__restrict__ stw,
+                                             const LineIO &io, long long l0)
+{
+    typedef Ops<A> O;
+    typedef Cx<A> C;
+    const int nc = cx.L, nk = nc / 2 + 1;
+    for (int w = threadIdx.x; w < cx.TL * nk; w += blockDim.x) {
+        int line, k;
+        cx.split(w, nk, line, k);
+        if (line >= cx.nl)
+            continue;
+        const long long l = l0 + line;
+        const long long base = (l / io.nlo) * io.out_hi + (l % io.nlo) * io.out_lo;
+        if (k == 0) {
+            C dc = O::fixdiv(O::load(z[cx.idx(line, 0)]), 2);
+            out[base] = O::pack(C{O::add(dc.r, dc.i), O::zero().r});
+            out[base + (long long)nc * io.out_el] = O::pack(C{O::sub(dc.r, dc.i), O::zero().r});
+            continue;
+        }
+        C a = O::load(z[cx.idx(line, k)]);
+        C bz = O::load(z[cx.idx(line, nc - k)]);
+        C b{bz.r, O::neg(bz.i)};
+        a = O::fixdiv(a, 2);
+        b = O::fixdiv(b, 2);
+        C f1 = O::cadd(a, b), f2 = O::csub(a, b);
+        C t = O::cmul(f2, O::load(stw[k - 1]));
+        if (2 * k != nc)
+            out[base + (long long)k * io.out_el] = O::pack(C{O::half_add(f1.r, t.r), O::half_add(f1.i, t.i)});
+        out[base + (long long)(nc - k) * io.out_el] = O::pack(C{O::half_sub(f1.r, t.r), O::half_sub(t.i, f1.i)});
+    }
+}
+
+// Hermitian merge before the half-length inverse transform, kiss_fftr.c:131-153.
+template <class A>
+__device__ __forceinline__ void c2r_prologue(const LineCtx<A> &cx, typename A::store2 *z,
+                                             const typename A::store2 *__restrict__ in,
+                                             const typename A::store2 *__restrict__ stw,
+                                             const LineIO &io, long long l0)
+{
+    typedef Ops<A> O;
+    typedef Cx<A> C;
+    const int nc = cx.L, nk = nc / 2 + 1;
+    for (int w = threadIdx.x; w < cx.TL * nk; w += blockDim.x) {
+        int line, k;
+        cx.split(w, nk, line, k);
+        if (line >= cx.nl)
+            continue;
+        const long long l = l0 + line;
+        const long long base = (l / io.nlo) * io.in_hi + (l % io.nlo) * io.in_lo;
+        if (k == 0) {
+            C x0 = O::load(in[base]), xn = O::load(in[base + (long long)nc * io.in_el]);
+            C d{O::add(x0.r, xn.r), O::sub(x0.r, xn.r)};
+            z[cx.idx(line, 0)] = O::pack(O::fixdiv(d, 2));
+            continue;
+        }
+        C a = O::load(in[base + (long long)k * io.in_el]);
+        C bz = O::load(in[base + (long long)(nc - k) * io.in_el]);
+        C b{bz.r, O::neg(bz.i)};
+        a = O::fixdiv(a, 2);
+        b = O::fixdiv(b, 2);
+        C e = O::cadd(a, b), d = O::csub(a, b);
+        C o = O::cmul(d, O::load(stw[k - 1]));
+        C lo = O::csub(e, o);
+        lo.i = O::neg(lo.i);
+        if (2 * k != nc)
+            z[cx.idx(line, k)] = O::pack(O::cadd(e, o));
+        z[cx.idx(line, nc - k)] = O::pack(lo);
+    }
+}
+
+// The line kernel.  Dynamic LDS: 2 * TL * L complex values.
+template <class A>
+__global__ void __launch_bounds__(256) k_lines(const typename A::store2 *__restrict__ in,
+                                               typename A::store2 *__restrict__ out,
+                                               const typename A::store2 *__restrict__ tw,
+                                               const typename A::store2 *__restrict__ stw,
+                                               LinePlan pl, LineIO io, int TL, int colmode, int prologue,
+                                               int epilogue)
+{
+    typedef typename A::store2 S;
+    extern __shared__ __align__(16) unsigned char kfa_smem[];
+    S *bufA = reinterpret_cast<S *>(kfa_smem);
+    S *bufB = bufA + (size_t)TL * pl.L;
+
+    const long long l0 = (long long)blockIdx.x * TL;
+    LineCtx<A> cx;
+    cx.TL = TL;
+    cx.L = pl.L;
+    cx.colmode = colmode;
+    cx.nl = (int)((io.nlines - l0) < TL ? (io.nlines - l0) : TL);
+
+    if (prologue == PRO_C2R) {
+        c2r_prologue<A>(cx, bufA, in, stw, io, l0);
+    } else {
+        const int L = pl.L;
+        for (int w = threadIdx.x; w < TL * L; w += blockDim.x) {
+            int line, t;
+            cx.split(w, L, line, t);
+            if (line >= cx.nl)
+                continue;
+            const long long l = l0 + line;
+            const long long base = (l / io.nlo) * io.in_hi + (l % io.nlo) * io.in_lo;
+            bufA[cx.idx(line, t)] = in[base + (long long)t * io.in_el];
+        }
+    }
+    __syncthreads();
+
+    S *src = bufA, *dst = bufB;
+    for (int s = 0; s < pl.nst; ++s) {
+        run_stage<A>(pl, pl.st[s], cx, src, dst, tw, l0, io.nlo);
+        __syncthreads();
+        S *t = src;
+        src = dst;
+        dst = t;
+    }
+
+    if (epilogue == EPI_R2C) {
+        r2c_epilogue<A>(cx, src, out, stw, io, l0);
+    } else {
+        const int L = pl.L;
+        for (int w = threadIdx.x; w < TL * L; w += blockDim.x) {
+            int line, t;
+            cx.split(w, L, line, t);
+            if (line >= cx.nl)
+                continue;
+            const long long l = l0 + line;
+            const long long base = (l / io.nlo) * io.out_hi + (l % io.nlo) * io.out_lo;
+            out[base + (long long)t * io.out_el] = src[cx.idx(line, t)];
+        }
+    }
+}
+
+}  // namespace kfa

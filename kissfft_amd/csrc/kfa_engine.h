@@ -1,0 +1,82 @@
+/*
+ * kfa_engine.h -- the thin C ABI between the C host layer (the files under kissfft_amd/csrc/host)
+ * and the HIP engine (kissfft_amd/csrc/engine/kfa_engine.hip).
+ *
+ * Internal to libkissfft-<type>.so: the symbols are hidden.  The public entry points are
+ * the reference API (include/kiss_fft*.h, include/kfc.h) and the batched API
+ * (include/kiss_fft_batch.h).
+ *
+ * Plain C types only.  Data pointers may be host memory (staged through HBM, the call is
+ * synchronous) or device/managed memory (used in place).  `stream` is a hipStream_t or
+ * NULL for the calling thread's own stream; `sync` != 0 waits for completion.
+ * Every entry returns 0 on success and a negative code on failure; kfa_last_error()
+ * gives the message for the calling thread.
+ */
+#ifndef KFA_ENGINE_H
+#define KFA_ENGINE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KFA_HIDDEN __attribute__((visibility("hidden")))
+
+enum { KFA_F32 = 0, KFA_Q15 = 1, KFA_Q31 = 2 };
+
+/* One 1-D complex plan as the host cfg holds it (struct kiss_fft_state,
+ * _kiss_fft_guts.h:27-32): length, direction, (p, m) radix pairs ending with m == 1,
+ * and the host twiddle table of nfft complex scalars. */
+typedef struct {
+    int type;
+    int nfft;
+    int inverse;
+    const int *factors;
+    const void *twiddles;
+} kfa_cplan;
+
+/* One real plan (struct kiss_fftr_state, kiss_fftr.c:12-19): the half-length complex
+ * plan plus nfft/4 split twiddles. */
+typedef struct {
+    kfa_cplan half;
+    const void *super_twiddles;
+} kfa_rplan;
+
+KFA_HIDDEN int kfa_c2c(const kfa_cplan *pl, const void *in, long long istride, long long idist,
+                       void *out, long long odist, long long howmany, void *stream, int sync);
+
+KFA_HIDDEN int kfa_r2c(const kfa_rplan *pl, const void *in, long long idist, void *out,
+                       long long odist, long long howmany, void *stream, int sync);
+
+KFA_HIDDEN int kfa_c2r(const kfa_rplan *pl, const void *in, long long idist, void *out,
+                       long long odist, long long howmany, void *stream, int sync);
+
+/* Complex N-D over `ndims` axes (dims[0] slowest), items dense and back to back. */
+KFA_HIDDEN int kfa_nd(int ndims, const int *dims, const kfa_cplan *axes, const void *in,
+                      void *out, long long howmany, void *stream, int sync);
+
+/* Real N-D: real transform on the last axis, complex N-D over the others. */
+KFA_HIDDEN int kfa_ndr(int ndims, const int *dims, const kfa_rplan *rpl, const kfa_cplan *axes,
+                       int inverse, const void *in, void *out, long long howmany, void *stream,
+                       int sync);
+
+/* Waits for `stream` (NULL: the calling thread's stream). */
+KFA_HIDDEN int kfa_stream_sync(void *stream);
+
+KFA_HIDDEN void kfa_cleanup(void);
+KFA_HIDDEN const char *kfa_last_error(void);
+KFA_HIDDEN int kfa_device_available(void);
+
+/* Counters for tests and the serving layer. */
+typedef struct {
+    long long plans_built;
+    long long plan_hits;
+    long long launches;
+    long long h2d_bytes;
+    long long d2h_bytes;
+} kfa_stats;
+KFA_HIDDEN void kfa_get_stats(kfa_stats *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

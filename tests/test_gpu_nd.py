@@ -1,0 +1,135 @@
+"""GPU parity of the N-D paths (kiss_fftnd / kiss_fftndr / kiss_fftndri / batched forms)."""
+import sys
+
+import numpy as np
+import pytest
+
+import kissfft_amd
+from helpers import ABS_FLOAT_BOUND, cplx, fixture_cases, float_bound, pairs, parse_dims, rel_l2
+from kissfft_amd import workloads as W
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def nd(lib, dims, x, inverse=False):
+    cfg = lib.fftnd_alloc(dims, inverse)
+    out = np.zeros_like(x)
+    lib.fftnd_batch(cfg, np.ascontiguousarray(x), out, x.shape[0])
+    return out
+
+
+def ndr(lib, dims, x):
+    cfg = lib.fftndr_alloc(dims, False)
+    out = np.zeros((x.shape[0],) + tuple(dims[:-1]) + (dims[-1] // 2 + 1, 2), lib.scalar)
+    lib.fftndr_batch(cfg, np.ascontiguousarray(x), out, x.shape[0])
+    return out
+
+
+def ndri(lib, dims, X):
+    cfg = lib.fftndr_alloc(dims, True)
+    out = np.zeros((X.shape[0],) + tuple(dims), lib.scalar)
+    lib.fftndri_batch(cfg, np.ascontiguousarray(X), out, X.shape[0])
+    return out
+
+
+@pytest.mark.parametrize("datatype,gname", [("int16_t", "golden_q15"), ("int32_t", "golden_q31")])
+def test_fixed_nd_matches_reference_fixtures(datatype, gname, request):
+    lib = kissfft_amd.load(datatype)
+    g = request.getfixturevalue(gname)
+    for stem, x, want in fixture_cases(g, "nd"):
+        _, dims, inv = stem.split("_")
+        dims = parse_dims(dims)
+        say("case", datatype, stem)
+        np.testing.assert_array_equal(nd(lib, dims, x, bool(int(inv))), want, err_msg=stem)
+        cfg = lib.fftnd_alloc(dims, bool(int(inv)))
+        one = np.zeros_like(x[0])
+        lib.kiss_fftnd(cfg, np.ascontiguousarray(x[0]), one)
+        np.testing.assert_array_equal(one, want[0], err_msg=stem + " single")
+    for stem, x, want in fixture_cases(g, "ndr"):
+        dims = parse_dims(stem.split("_")[1])
+        say("case", datatype, stem)
+        np.testing.assert_array_equal(ndr(lib, dims, x), want, err_msg=stem)
+    for stem, X, want in fixture_cases(g, "ndri"):
+        dims = parse_dims(stem.split("_")[1])
+        say("case", datatype, stem)
+        np.testing.assert_array_equal(ndri(lib, dims, X), want, err_msg=stem)
+
+
+def test_q15_nd_bit_exact_vs_oracle_larger_shapes():
+    lib = kissfft_amd.load("int16_t")
+    O = Oracle("int16_t")
+    rng = np.random.default_rng(41)
+    for dims in [(64, 96, 120), (32, 1024), (3, 5, 7, 9), (1009,)]:
+        x = rng.integers(-32768, 32768, (1,) + dims + (2,)).astype(np.int16)
+        say("case q15 nd", dims)
+        np.testing.assert_array_equal(nd(lib, dims, x), O.fftnd(x, dims))
+    for dims in [(64, 100), (16, 12, 10), (1024, 1000)]:
+        x = rng.integers(-32768, 32768, (1,) + dims).astype(np.int16)
+        say("case q15 ndr", dims)
+        np.testing.assert_array_equal(ndr(lib, dims, x), O.fftndr(x, dims))
+        X = rng.integers(-32768, 32768, (1,) + dims[:-1] + (dims[-1] // 2 + 1, 2)).astype(np.int16)
+        np.testing.assert_array_equal(ndri(lib, dims, X), O.fftndri(X, dims))
+
+
+def test_float_nd_workload(golden_float):
+    # 16 transforms of 64x96x120, re/im uniform on [-1,1), seed 4
+    lib = kissfft_amd.load("float")
+    dims = (64, 96, 120)
+    x = W.complex_pm1(4, int(np.prod(dims)), 16).reshape((16,) + dims + (2,))
+    say("case float nd 64x96x120 x16")
+    got = cplx(nd(lib, dims, x))
+    err = rel_l2(got, np.fft.fftn(cplx(x), axes=(1, 2, 3)))
+    say("rel-L2", err, "bound", float_bound(golden_float, "nd_64x96x120"))
+    assert err <= float_bound(golden_float, "nd_64x96x120")
+
+
+def test_float_ndr_workload(golden_float):
+    # 32 transforms of 1024x1000 real N(0,1), seed 5 (the reference's alloc rejects this shape)
+    lib = kissfft_amd.load("float")
+    dims = (1024, 1000)
+    x = W.normal(5, int(np.prod(dims)) * 32).reshape((32,) + dims)
+    say("case float ndr 1024x1000 x32")
+    X = ndr(lib, dims, x)
+    want = np.fft.rfftn(x.astype(np.float64), axes=(1, 2))
+    err = rel_l2(cplx(X), want)
+    say("rel-L2", err, "bound", float_bound(golden_float, "ndr_1024x1000"))
+    assert err <= float_bound(golden_float, "ndr_1024x1000")
+    y = ndri(lib, dims, X)
+    rt = rel_l2(y, np.prod(dims) * x.astype(np.float64))
+    say("round trip", rt)
+    assert rt <= 2 * ABS_FLOAT_BOUND
+
+
+def test_float_nd_shapes():
+    lib = kissfft_amd.load("float")
+    rng = np.random.default_rng(42)
+    for dims in [(2, 3), (4, 4, 4), (7, 11), (2, 3, 4, 5), (1, 8), (5,), (30, 1, 6)]:
+        x = rng.uniform(-1, 1, (2,) + dims + (2,)).astype(np.float32)
+        for inv in (False, True):
+            got = cplx(nd(lib, dims, x, inv))
+            axes = tuple(range(1, len(dims) + 1))
+            want = np.fft.ifftn(cplx(x), axes=axes) * np.prod(dims) if inv else np.fft.fftn(cplx(x), axes=axes)
+            assert rel_l2(got, want) <= ABS_FLOAT_BOUND, (dims, inv)
+    for dims in [(4, 6), (3, 5, 8), (12,), (2, 2, 2, 2)]:
+        x = rng.standard_normal((2,) + dims).astype(np.float32)
+        X = ndr(lib, dims, x)
+        axes = tuple(range(1, len(dims) + 1))
+        assert rel_l2(cplx(X), np.fft.rfftn(x.astype(np.float64), axes=axes)) <= ABS_FLOAT_BOUND, dims
+        y = ndri(lib, dims, X)
+        assert rel_l2(y, np.prod(dims) * x.astype(np.float64)) <= 2 * ABS_FLOAT_BOUND, dims
+
+
+def test_nd_in_place_single_call():
+    lib = kissfft_amd.load("float")
+    dims = (8, 12, 10)
+    x = W.complex_pm1(43, int(np.prod(dims)), 1).reshape(dims + (2,))
+    want = np.fft.fftn(cplx(x))
+    cfg = lib.fftnd_alloc(dims)
+    lib.kiss_fftnd(cfg, x, x)
+    assert rel_l2(cplx(x), want) <= ABS_FLOAT_BOUND
