@@ -71,12 +71,26 @@ class Cfg:
             self.ptr = 0
 
 
+def _share_hip_runtime() -> None:
+    """Load PyTorch (when installed) before the library.
+
+    PyTorch ships its own libamdhip64 with the same soname as ROCm's.  Loaded first, it is
+    the one copy of the HIP runtime in the process and the library binds to it, so torch
+    tensors' device pointers are valid here.  Loaded the other way round, the process would
+    hold two HIP runtimes, and the second one finds no device."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 class Library:
     """ctypes view of one libkissfft-<datatype>.so."""
 
     def __init__(self, datatype: str = "float"):
         if datatype not in SCALARS:
             raise ValueError(f"datatype must be one of {list(SCALARS)}")
+        _share_hip_runtime()
         path = _build.lib_path(datatype)
         if not os.path.exists(path):
             raise KissFFTError(f"{path} is not built: run `python -m kissfft_amd.build` "

@@ -26,6 +26,8 @@
 #include <vector>
 
 #include "../kfa_engine.h"
+#include "kfa_fast.hpp"
+#include "kfa_fast_reg.hpp"
 #include "kfa_lines.hpp"
 
 namespace kfa {
@@ -63,14 +65,24 @@ static const size_t kLdsMax = 160 * 1024;
 // ------------------------------------------------------------------------------------
 // device plans
 // ------------------------------------------------------------------------------------
+// The fast variant chosen for one transform kind of a plan, with its stage table.
+struct FastChoice {
+    bool decided = false;
+    const FastVariant *v = nullptr;
+    void *d_stw = nullptr;
+};
+
 struct DevPlan {
     int type = 0, n = 0, inverse = 0;
     std::vector<int> radix;  // execution order (innermost factor first)
+    std::vector<unsigned char> h_tw;  // host copy of the reference twiddle table
     void *d_tw = nullptr;    // n complex
     bool single = true;
     LinePlan lp{};           // single pass
     int L1 = 0, Q = 0;       // four-step: L1 * Q == n
     LinePlan lp1{}, lp2{};
+    std::mutex mu;           // guards `fast`
+    FastChoice fast[3];      // per FK_C2C / FK_R2C / FK_C2R
 };
 
 struct RealTables {
@@ -175,6 +187,7 @@ static int build_cplan(const kfa_cplan *hp, int dev, std::shared_ptr<DevPlan> &o
         fill_stages(P->lp2, P->radix, best, K, n, P->Q, 1, P->L1, P->L1, 1, P->inverse);
     }
     const size_t tw_bytes = cpx_bytes(hp->type) * (size_t)n;
+    P->h_tw.assign((const unsigned char *)hp->twiddles, (const unsigned char *)hp->twiddles + tw_bytes);
     KFA_HIP(hipMalloc(&P->d_tw, tw_bytes));
     KFA_HIP(hipMemcpy(P->d_tw, hp->twiddles, tw_bytes, hipMemcpyHostToDevice));
     ++g_plans_built;
@@ -219,6 +232,128 @@ static int build_rtab(const kfa_rplan *hp, int dev, std::shared_ptr<RealTables> 
     }
     C.rtabs[key] = R;
     out = R;
+    return 0;
+}
+
+// ------------------------------------------------------------------------------------
+// fast variants: per-stage twiddle tables and first-use selection
+// ------------------------------------------------------------------------------------
+static bool env_flag(const char *name, bool dflt)
+{
+    const char *v = getenv(name);
+    if (!v || !*v)
+        return dflt;
+    return !(v[0] == '0' || v[0] == 'n' || v[0] == 'N' || v[0] == 'f' || v[0] == 'F');
+}
+
+// Gathers the stage tables of schedule `rad` from the full-length table: entry (u, r) of
+// stage s is table[r*u*P_s]; then table[P_s*m_s] and table[2*P_s*m_s].
+static int make_stage_table(const DevPlan &P, const std::vector<int> &rad, void **d_out)
+{
+    const size_t cb = cpx_bytes(P.type);
+    const long long n = P.n;
+    std::vector<unsigned char> h;
+    long long m = 1;
+    auto put = [&](long long idx) {
+        const unsigned char *src = P.h_tw.data() + cb * (size_t)(idx % n);
+        h.insert(h.end(), src, src + cb);
+    };
+    for (int p : rad) {
+        const long long Pst = n / (p * m);
+        for (long long u = 0; u < m; ++u)
+            for (int r = 1; r < p; ++r)
+                put((long long)r * u * Pst);
+        put(Pst * m);
+        put(2 * Pst * m);
+        m *= p;
+    }
+    KFA_HIP(hipMalloc(d_out, h.size()));
+    KFA_HIP(hipMemcpy(*d_out, h.data(), h.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// Chooses (once per plan and kind) the fastest registered variant, or none.
+static int choose_fast(DevPlan &P, int kind, hipStream_t s, const FastChoice **out)
+{
+    std::lock_guard<std::mutex> g(P.mu);
+    FastChoice &fc = P.fast[kind];
+    *out = &fc;
+    if (fc.decided)
+        return 0;
+    fc.decided = true;
+    if (!P.single || !env_flag("KISSFFT_AMD_FAST", true))
+        return 0;
+    std::vector<const FastVariant *> cand;
+    for (const FastVariant &v : fast_registry()) {
+        if (v.type != P.type || v.kind != kind || v.n != P.n || v.inverse != P.inverse)
+            continue;
+        if (P.type != KFA_F32 && v.radices != P.radix)
+            continue;  // fixed point must keep the reference schedule
+        cand.push_back(&v);
+    }
+    if (cand.empty())
+        return 0;
+    std::vector<void *> tabs(cand.size(), nullptr);
+    for (size_t i = 0; i < cand.size(); ++i) {
+        int rc = make_stage_table(P, cand[i]->radices, &tabs[i]);
+        if (rc)
+            return rc;
+    }
+    size_t best = 0;
+    if (cand.size() > 1 && env_flag("KISSFFT_AMD_TUNE", true)) {
+        // time every candidate on a scratch batch of this length
+        const long long lines = P.n >= 2048 ? 8192 : 32768;
+        const size_t cb = cpx_bytes(P.type);
+        const size_t bytes = cb * (size_t)(P.n + 1) * (size_t)lines * 2;
+        void *bin = nullptr, *bout = nullptr, *brtw = nullptr;
+        KFA_HIP(hipMallocAsync(&bin, bytes, s));
+        KFA_HIP(hipMallocAsync(&bout, bytes, s));
+        KFA_HIP(hipMallocAsync(&brtw, cb * (size_t)(P.n / 2 + 1), s));
+        KFA_HIP(hipMemsetAsync(bin, 0, bytes, s));
+        KFA_HIP(hipMemsetAsync(brtw, 0, cb * (size_t)(P.n / 2 + 1), s));
+        hipEvent_t e0, e1;
+        KFA_HIP(hipEventCreate(&e0));
+        KFA_HIP(hipEventCreate(&e1));
+        float best_ms = 0;
+        for (size_t i = 0; i < cand.size(); ++i) {
+            FastArgs a{};
+            a.in = bin;
+            a.out = bout;
+            a.stw = tabs[i];
+            a.rtw = brtw;
+            a.nlines = lines;
+            a.in_hi = kind == FK_C2R ? P.n + 1 : P.n;
+            a.in_el = 1;
+            a.out_hi = kind == FK_R2C ? P.n + 1 : P.n;
+            a.out_el = 1;
+            cand[i]->launch(a, s);  // warm-up
+            KFA_HIP(hipEventRecord(e0, s));
+            for (int r = 0; r < 3; ++r)
+                cand[i]->launch(a, s);
+            KFA_HIP(hipEventRecord(e1, s));
+            KFA_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            KFA_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (i == 0 || ms < best_ms) {
+                best_ms = ms;
+                best = i;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipFreeAsync(bin, s);
+        (void)hipFreeAsync(bout, s);
+        (void)hipFreeAsync(brtw, s);
+        KFA_HIP(hipStreamSynchronize(s));
+    }
+    for (size_t i = 0; i < cand.size(); ++i)
+        if (i != best)
+            (void)hipFree(tabs[i]);
+    fc.v = cand[best];
+    fc.d_stw = tabs[best];
+    if (env_flag("KISSFFT_AMD_VERBOSE", false))
+        fprintf(stderr, "[kissfft_amd] n=%d kind=%d inverse=%d: fast variant %s\n", P.n, kind, P.inverse,
+                fc.v->name);
     return 0;
 }
 
@@ -386,13 +521,30 @@ static int launch_lines(int type, const LineLaunch &ll, hipStream_t s)
 
 // Runs `howmany` complex transforms of plan P between two device buffers.
 // in: element t of item b at in + (b*idist + t*istride); out: out + (b*odist + t*ostride).
-static int run_c2c(const DevPlan &P, const void *in, long long istride, long long idist, void *out,
+static int run_c2c(DevPlan &P, const void *in, long long istride, long long idist, void *out,
                    long long ostride, long long odist, long long howmany, hipStream_t s)
 {
     const int n = P.n;
     if (howmany <= 0 || n <= 0)
         return 0;
     if (P.single) {
+        const FastChoice *fc = nullptr;
+        int rc = choose_fast(P, FK_C2C, s, &fc);
+        if (rc)
+            return rc;
+        if (fc->v) {
+            FastArgs a{};
+            a.in = in;
+            a.out = out;
+            a.stw = fc->d_stw;
+            a.nlines = howmany;
+            a.in_hi = idist;
+            a.in_el = istride;
+            a.out_hi = odist;
+            a.out_el = ostride;
+            ++g_launches;
+            return fc->v->launch(a, s) ? fail(-3, "fast kernel launch failed") : 0;
+        }
         LineLaunch ll{};
         ll.in = in;
         ll.out = out;
@@ -447,6 +599,53 @@ static int run_c2c(const DevPlan &P, const void *in, long long istride, long lon
     }
     (void)hipFreeAsync(tmp, s);
     return rc;
+}
+
+// Real rows: `howmany` lines, distances in complex units (real side counted in pairs).
+static int run_real_rows(DevPlan &H, const RealTables &R, int inverse, const void *in, long long in_hi,
+                         void *out, long long out_hi, long long howmany, hipStream_t s)
+{
+    if (howmany <= 0)
+        return 0;
+    if (!H.single)
+        return fail(-2, "real transform of %d points: half length exceeds one LDS line (not yet supported)",
+                    2 * H.n);
+    const FastChoice *fc = nullptr;
+    int rc = choose_fast(H, inverse ? FK_C2R : FK_R2C, s, &fc);
+    if (rc)
+        return rc;
+    if (fc->v) {
+        FastArgs a{};
+        a.in = in;
+        a.out = out;
+        a.stw = fc->d_stw;
+        a.rtw = R.d_stw;
+        a.nlines = howmany;
+        a.in_hi = in_hi;
+        a.in_el = 1;
+        a.out_hi = out_hi;
+        a.out_el = 1;
+        ++g_launches;
+        return fc->v->launch(a, s) ? fail(-3, "fast kernel launch failed") : 0;
+    }
+    LineLaunch ll{};
+    ll.in = in;
+    ll.out = out;
+    ll.tw = H.d_tw;
+    ll.stw = R.d_stw;
+    ll.pl = &H.lp;
+    ll.io.nlines = howmany;
+    ll.io.nlo = 1;
+    ll.io.in_hi = in_hi;
+    ll.io.in_el = 1;
+    ll.io.out_hi = out_hi;
+    ll.io.out_el = 1;
+    ll.colmode = 0;
+    if (inverse)
+        ll.prologue = PRO_C2R;
+    else
+        ll.epilogue = EPI_R2C;
+    return launch_lines(H.type, ll, s);
 }
 
 static int begin(hipStream_t *s, void *stream, int *dev)
@@ -570,45 +769,14 @@ static int real_common(const kfa_rplan *pl, const void *in, long long idist, voi
     if (rc == 0)
         rc = stage_out(out, inverse ? real_bytes_out : cpx_bytes_out, s, O);
     if (rc == 0) {
-        if (!P->single) {
-            rc = fail(-2, "real transform of %lld points: half length exceeds one LDS line (not yet supported)",
-                      2 * nc);
-        } else {
-            LineLaunch ll{};
-            ll.tw = P->d_tw;
-            ll.stw = R->d_stw;
-            ll.pl = &P->lp;
-            ll.io.nlines = howmany;
-            ll.io.nlo = 1;
-            ll.colmode = 0;
-            if (!inverse) {
-                if (idist & 1) {
-                    rc = fail(-2, "real input distance must be even (complex-pair loads)");
-                } else {
-                    ll.in = I.dev;
-                    ll.out = O.dev;
-                    ll.io.in_hi = idist / 2;  // in complex pairs
-                    ll.io.in_el = 1;
-                    ll.io.out_hi = odist;
-                    ll.io.out_el = 1;
-                    ll.epilogue = EPI_R2C;
-                    rc = launch_lines(pl->half.type, ll, s);
-                }
-            } else {
-                if (odist & 1) {
-                    rc = fail(-2, "real output distance must be even (complex-pair stores)");
-                } else {
-                    ll.in = I.dev;
-                    ll.out = O.dev;
-                    ll.io.in_hi = idist;
-                    ll.io.in_el = 1;
-                    ll.io.out_hi = odist / 2;
-                    ll.io.out_el = 1;
-                    ll.prologue = PRO_C2R;
-                    rc = launch_lines(pl->half.type, ll, s);
-                }
-            }
-        }
+        if (!inverse && (idist & 1))
+            rc = fail(-2, "real input distance must be even (complex-pair loads)");
+        else if (inverse && (odist & 1))
+            rc = fail(-2, "real output distance must be even (complex-pair stores)");
+        else if (!inverse)
+            rc = run_real_rows(*P, *R, 0, I.dev, idist / 2, O.dev, odist, howmany, s);
+        else
+            rc = run_real_rows(*P, *R, 1, I.dev, idist, O.dev, odist / 2, howmany, s);
         if (rc == 0)
             rc = finish_out(O, s);
     }
@@ -630,7 +798,7 @@ int kfa_c2r(const kfa_rplan *pl, const void *in, long long idist, void *out, lon
 }
 
 // One axis of an N-D array batch, dims[k] long, between device buffers (in may equal out).
-static int run_axis(const DevPlan &P, const void *in, void *out, long long outer, long long inner,
+static int run_axis(DevPlan &P, const void *in, void *out, long long outer, long long inner,
                     hipStream_t s)
 {
     const long long len = P.n;
@@ -733,22 +901,9 @@ int kfa_ndr(int ndims, const int *dims, const kfa_rplan *rpl, const kfa_cplan *a
     if (rc == 0)
         rc = stage_out(out, inverse ? real_bytes : cpx_bytes_, s, O);
     if (rc == 0) {
-        LineLaunch ll{};
-        ll.tw = H->d_tw;
-        ll.stw = R->d_stw;
-        ll.pl = &H->lp;
-        ll.io.nlines = rows * howmany;
-        ll.io.nlo = 1;
         if (!inverse) {
             // kiss_fftndr.c:99-109: real rows first, then the complex axes per bin column
-            ll.in = I.dev;
-            ll.out = O.dev;
-            ll.io.in_hi = nreal / 2;
-            ll.io.in_el = 1;
-            ll.io.out_hi = nbins;
-            ll.io.out_el = 1;
-            ll.epilogue = EPI_R2C;
-            rc = launch_lines(rpl->half.type, ll, s);
+            rc = run_real_rows(*H, *R, 0, I.dev, nreal / 2, O.dev, nbins, rows * howmany, s);
             long long outer = howmany, inner = rows * nbins;
             for (int k = 0; k + 1 < ndims && rc == 0; ++k) {
                 inner /= dims[k];
@@ -771,16 +926,8 @@ int kfa_ndr(int ndims, const int *dims, const kfa_rplan *rpl, const kfa_cplan *a
             }
             if (ndims == 1)
                 KFA_HIP(hipMemcpyAsync(tmp, I.dev, tb, hipMemcpyDeviceToDevice, s));
-            if (rc == 0) {
-                ll.in = tmp;
-                ll.out = O.dev;
-                ll.io.in_hi = nbins;
-                ll.io.in_el = 1;
-                ll.io.out_hi = nreal / 2;
-                ll.io.out_el = 1;
-                ll.prologue = PRO_C2R;
-                rc = launch_lines(rpl->half.type, ll, s);
-            }
+            if (rc == 0)
+                rc = run_real_rows(*H, *R, 1, tmp, nbins, O.dev, nreal / 2, rows * howmany, s);
             (void)hipFreeAsync(tmp, s);
         }
         if (rc == 0)

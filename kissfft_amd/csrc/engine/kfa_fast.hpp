@@ -1,0 +1,348 @@
+// kfa_fast.hpp -- compile-time specialised single-pass Stockham kernels for hot sizes.
+//
+// Same stage arithmetic as kfa_lines.hpp (butterfly b = u*P + o reads u*P*p + r*P + o,
+// writes b + q*(N/p), twiddles table[r*u*P]), but with the radix schedule, the line length
+// and the thread mapping fixed at compile time, so all index arithmetic folds into
+// constants and every loop unrolls.  Data moves HBM -> registers (the first stage reads its
+// inputs straight from HBM: for m == 1 the reads r*(N/p) + b are coalesced), then
+// registers -> LDS -> registers between stages (ping-pong buffers, one barrier per stage),
+// and the last stage writes registers -> HBM (outputs b + q*(N/p), coalesced).  The real
+// transforms fuse the split (forward) or merge (inverse) of kiss_fftr.c into the same pass.
+//
+// Twiddles come from a per-stage table built on the host by gathering the reference's
+// full-length table (entry (u, r) of stage s is table[r*u*P_s]; the last two entries are
+// table[P_s*m_s] and table[2*P_s*m_s], the fixed rotations of radix 3 and 5), so fixed
+// point uses exactly the values kf_bfly* uses.  Float stages use their own radix-p DFTs
+// (the decimation is the same, the butterfly algebra is free); fixed-point stages use the
+// reference-order butterflies of kfa_arith.hpp.
+#pragma once
+
+#include <utility>
+
+#include "kfa_arith.hpp"
+#include "kfa_lines.hpp"
+
+namespace kfa {
+
+enum { FK_C2C = 0, FK_R2C = 1, FK_C2R = 2 };
+
+template <int... R>
+struct Radices {
+    static constexpr int K = sizeof...(R);
+    static constexpr int r[K] = {R...};
+    static constexpr int p(int s) { return r[s]; }
+    static constexpr int m(int s)
+    {
+        int v = 1;
+        for (int i = 0; i < s; ++i)
+            v *= r[i];
+        return v;
+    }
+    static constexpr int N() { return m(K); }
+    static constexpr int tw_off(int s)
+    {
+        int o = 0;
+        for (int i = 0; i < s; ++i)
+            o += m(i) * (r[i] - 1) + 2;
+        return o;
+    }
+    static constexpr int tw_len() { return tw_off(K); }
+};
+
+struct FastArgs {
+    const void *in;
+    void *out;
+    const void *stw;  // per-stage twiddle tables
+    const void *rtw;  // real split twiddles (nfft/4 entries)
+    long long nlines;
+    long long in_hi, in_el;
+    long long out_hi, out_el;
+};
+
+// ------------------------------------------------------------------------------------
+// Float radix-p DFTs on registers.  Forward uses exp(-2*pi*i/p); INV the conjugate.
+// ------------------------------------------------------------------------------------
+typedef Cx<F32> Cf;
+
+__device__ __forceinline__ Cf cf_add(Cf a, Cf b) { return Cf{a.r + b.r, a.i + b.i}; }
+__device__ __forceinline__ Cf cf_sub(Cf a, Cf b) { return Cf{a.r - b.r, a.i - b.i}; }
+// multiply by -i (forward) or +i (inverse)
+template <bool INV>
+__device__ __forceinline__ Cf cf_rot(Cf a)
+{
+    return INV ? Cf{-a.i, a.r} : Cf{a.i, -a.r};
+}
+
+template <int P, bool INV>
+struct Dft;
+
+template <bool INV>
+struct Dft<2, INV> {
+    __device__ __forceinline__ static void run(Cf *v)
+    {
+        Cf a = v[0], b = v[1];
+        v[0] = cf_add(a, b);
+        v[1] = cf_sub(a, b);
+    }
+};
+
+template <bool INV>
+struct Dft<4, INV> {
+    __device__ __forceinline__ static void run(Cf *v)
+    {
+        Cf t0 = cf_add(v[0], v[2]), t1 = cf_sub(v[0], v[2]);
+        Cf t2 = cf_add(v[1], v[3]), t3 = cf_rot<INV>(cf_sub(v[1], v[3]));
+        v[0] = cf_add(t0, t2);
+        v[2] = cf_sub(t0, t2);
+        v[1] = cf_add(t1, t3);
+        v[3] = cf_sub(t1, t3);
+    }
+};
+
+template <bool INV>
+struct Dft<3, INV> {
+    __device__ __forceinline__ static void run(Cf *v)
+    {
+        const float s3 = 0.866025403784438646763723170752936183f;
+        Cf t = cf_add(v[1], v[2]);
+        Cf d = cf_sub(v[1], v[2]);
+        Cf mid{v[0].r - 0.5f * t.r, v[0].i - 0.5f * t.i};
+        Cf sd = cf_rot<INV>(Cf{s3 * d.r, s3 * d.i});
+        v[0] = cf_add(v[0], t);
+        v[1] = cf_add(mid, sd);
+        v[2] = cf_sub(mid, sd);
+    }
+};
+
+template <bool INV>
+struct Dft<5, INV> {
+    __device__ __forceinline__ static void run(Cf *v)
+    {
+        const float c1 = 0.309016994374947424102293417182819059f;   // cos(2pi/5)
+        const float c2 = -0.809016994374947424102293417182819059f;  // cos(4pi/5)
+        const float s1 = 0.951056516295153572116439333379382143f;   // sin(2pi/5)
+        const float s2 = 0.587785252292473129168705954639072769f;   // sin(4pi/5)
+        Cf t1 = cf_add(v[1], v[4]), d1 = cf_sub(v[1], v[4]);
+        Cf t2 = cf_add(v[2], v[3]), d2 = cf_sub(v[2], v[3]);
+        Cf x0 = v[0];
+        Cf a1{x0.r + c1 * t1.r + c2 * t2.r, x0.i + c1 * t1.i + c2 * t2.i};
+        Cf a2{x0.r + c2 * t1.r + c1 * t2.r, x0.i + c2 * t1.i + c1 * t2.i};
+        Cf b1 = cf_rot<INV>(Cf{s1 * d1.r + s2 * d2.r, s1 * d1.i + s2 * d2.i});
+        Cf b2 = cf_rot<INV>(Cf{s2 * d1.r - s1 * d2.r, s2 * d1.i - s1 * d2.i});
+        v[0] = Cf{x0.r + t1.r + t2.r, x0.i + t1.i + t2.i};
+        v[1] = cf_add(a1, b1);
+        v[4] = cf_sub(a1, b1);
+        v[2] = cf_add(a2, b2);
+        v[3] = cf_sub(a2, b2);
+    }
+};
+
+template <bool INV>
+struct Dft<8, INV> {
+    __device__ __forceinline__ static void run(Cf *v)
+    {
+        const float h = 0.707106781186547524400844362104849039f;
+        Cf e[4] = {v[0], v[2], v[4], v[6]};
+        Cf o[4] = {v[1], v[3], v[5], v[7]};
+        Dft<4, INV>::run(e);
+        Dft<4, INV>::run(o);
+        // o[k] *= W8^k
+        Cf o1 = INV ? Cf{h * (o[1].r - o[1].i), h * (o[1].r + o[1].i)}
+                    : Cf{h * (o[1].r + o[1].i), h * (o[1].i - o[1].r)};
+        Cf o2 = cf_rot<INV>(o[2]);
+        Cf o3 = INV ? Cf{-h * (o[3].r + o[3].i), h * (o[3].r - o[3].i)}
+                    : Cf{h * (o[3].i - o[3].r), -h * (o[3].r + o[3].i)};
+        v[0] = cf_add(e[0], o[0]);
+        v[4] = cf_sub(e[0], o[0]);
+        v[1] = cf_add(e[1], o1);
+        v[5] = cf_sub(e[1], o1);
+        v[2] = cf_add(e[2], o2);
+        v[6] = cf_sub(e[2], o2);
+        v[3] = cf_add(e[3], o3);
+        v[7] = cf_sub(e[3], o3);
+    }
+};
+
+// ------------------------------------------------------------------------------------
+// one stage
+// ------------------------------------------------------------------------------------
+template <class A, bool INV, int N, int T, int p, int m, int TWO, bool GSRC, bool GDST>
+__device__ __forceinline__ void fast_stage(const typename A::store2 *__restrict__ src, long long src_el,
+                                           typename A::store2 *__restrict__ dst, long long dst_el,
+                                           const typename A::store2 *__restrict__ stw, int t, bool valid)
+{
+    typedef Ops<A> O;
+    typedef Cx<A> C;
+    constexpr int NB = N / p;
+    constexpr int P = N / (p * m);
+    constexpr int J = (NB + T - 1) / T;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int b = t + j * T;
+        if ((NB % T) != 0 && b >= NB)
+            break;
+        const int u = b / P, o = b - (b / P) * P;
+        C v[p];
+#pragma unroll
+        for (int r = 0; r < p; ++r) {
+            const int idx = u * P * p + r * P + o;
+            if (GSRC)
+                v[r] = valid ? O::load(src[idx * src_el]) : O::zero();
+            else
+                v[r] = O::load(src[idx]);
+        }
+        if constexpr (A::fixed) {
+            C w[p > 1 ? p - 1 : 1];
+#pragma unroll
+            for (int r = 1; r < p; ++r)
+                w[r - 1] = O::load(stw[TWO + u * (p - 1) + r - 1]);
+            const C ya = O::load(stw[TWO + m * (p - 1)]);
+            const C yb = O::load(stw[TWO + m * (p - 1) + 1]);
+            if constexpr (p == 2)
+                bfly2<A>(v, w);
+            else if constexpr (p == 3)
+                bfly3<A>(v, w, ya);
+            else if constexpr (p == 4)
+                bfly4<A>(v, w, INV);
+            else
+                bfly5<A>(v, w, ya, yb);
+            (void)yb;
+        } else {
+            if constexpr (m > 1) {
+#pragma unroll
+                for (int r = 1; r < p; ++r)
+                    v[r] = O::cmul(v[r], O::load(stw[TWO + u * (p - 1) + r - 1]));
+            }
+            Dft<p, INV>::run(v);
+        }
+#pragma unroll
+        for (int q = 0; q < p; ++q) {
+            const int idx = b + q * NB;
+            if (GDST) {
+                if (valid)
+                    dst[idx * dst_el] = O::pack(v[q]);
+            } else {
+                dst[idx] = O::pack(v[q]);
+            }
+        }
+    }
+}
+
+template <class A, bool INV, int T, int KIND, class RS, int S>
+__device__ __forceinline__ void fast_run_one(typename A::store2 *bufA, typename A::store2 *bufB,
+                                             const typename A::store2 *gin, long long in_el,
+                                             typename A::store2 *gout, long long out_el,
+                                             const typename A::store2 *stw, int t, bool valid)
+{
+    typedef typename A::store2 Sx;
+    constexpr int K = RS::K;
+    static_assert(K >= 1, "empty schedule");
+    constexpr int N = RS::N();
+    constexpr bool GSRC = (S == 0) && (KIND != FK_C2R);
+    constexpr bool GDST = (S == K - 1) && (KIND != FK_R2C);
+    const Sx *src = GSRC ? gin : ((S % 2 == 0) ? bufA : bufB);
+    Sx *dst = GDST ? gout : ((S % 2 == 0) ? bufB : bufA);
+    fast_stage<A, INV, N, T, RS::p(S), RS::m(S), RS::tw_off(S), GSRC, GDST>(src, GSRC ? in_el : 1, dst,
+                                                                           GDST ? out_el : 1, stw, t, valid);
+    if constexpr (!GDST)
+        __syncthreads();
+}
+
+template <class A, bool INV, int T, int KIND, class RS, int... S>
+__device__ __forceinline__ void fast_run_all(std::integer_sequence<int, S...>, typename A::store2 *bufA,
+                                             typename A::store2 *bufB, const typename A::store2 *gin,
+                                             long long in_el, typename A::store2 *gout, long long out_el,
+                                             const typename A::store2 *stw, int t, bool valid)
+{
+    (fast_run_one<A, INV, T, KIND, RS, S>(bufA, bufB, gin, in_el, gout, out_el, stw, t, valid), ...);
+}
+
+// Dynamic LDS: 2 * LPB * N complex values.
+template <class A, int KIND, bool INV, int T, int LPB, class RS>
+__global__ void __launch_bounds__(T *LPB) k_fast(FastArgs a)
+{
+    typedef typename A::store2 S;
+    typedef Ops<A> O;
+    typedef Cx<A> C;
+    constexpr int N = RS::N();
+    constexpr int K = RS::K;
+    extern __shared__ __align__(16) unsigned char kfa_fast_smem[];
+    const int line = threadIdx.x / T;
+    const int t = threadIdx.x - line * T;
+    const long long gl = (long long)blockIdx.x * LPB + line;
+    const bool valid = gl < a.nlines;
+    S *bufA = reinterpret_cast<S *>(kfa_fast_smem) + line * N;
+    S *bufB = reinterpret_cast<S *>(kfa_fast_smem) + (LPB + line) * N;
+    const S *gin = reinterpret_cast<const S *>(a.in) + (valid ? gl * a.in_hi : 0);
+    S *gout = reinterpret_cast<S *>(a.out) + (valid ? gl * a.out_hi : 0);
+    const S *stw = reinterpret_cast<const S *>(a.stw);
+    const S *rtw = reinterpret_cast<const S *>(a.rtw);
+
+    if constexpr (KIND == FK_C2R) {
+        // Hermitian merge into bufA (kiss_fftr.c:131-153); bins read from HBM
+        constexpr int NK = N / 2 + 1;
+        constexpr int JK = (NK + T - 1) / T;
+#pragma unroll
+        for (int j = 0; j < JK; ++j) {
+            const int k = t + j * T;
+            if (k >= NK)
+                break;
+            if (k == 0) {
+                C x0 = valid ? O::load(gin[0]) : O::zero();
+                C xn = valid ? O::load(gin[(long long)N * a.in_el]) : O::zero();
+                C d{O::add(x0.r, xn.r), O::sub(x0.r, xn.r)};
+                bufA[0] = O::pack(O::fixdiv(d, 2));
+                continue;
+            }
+            C xa = valid ? O::load(gin[(long long)k * a.in_el]) : O::zero();
+            C bz = valid ? O::load(gin[(long long)(N - k) * a.in_el]) : O::zero();
+            C xb{bz.r, O::neg(bz.i)};
+            xa = O::fixdiv(xa, 2);
+            xb = O::fixdiv(xb, 2);
+            C e = O::cadd(xa, xb), d = O::csub(xa, xb);
+            C o = O::cmul(d, O::load(rtw[k - 1]));
+            C lo = O::csub(e, o);
+            lo.i = O::neg(lo.i);
+            if (2 * k != N)
+                bufA[k] = O::pack(O::cadd(e, o));
+            bufA[N - k] = O::pack(lo);
+        }
+        __syncthreads();
+    }
+
+    fast_run_all<A, INV, T, KIND, RS>(std::make_integer_sequence<int, K>{}, bufA, bufB, gin, a.in_el, gout,
+                                      a.out_el, stw, t, valid);
+
+    if constexpr (KIND == FK_R2C) {
+        // split (kiss_fftr.c:88-116) from the LDS buffer the last stage wrote
+        const S *z = (K % 2 == 0) ? bufA : bufB;
+        constexpr int NK = N / 2 + 1;
+        constexpr int JK = (NK + T - 1) / T;
+        if (!valid)
+            return;
+#pragma unroll
+        for (int j = 0; j < JK; ++j) {
+            const int k = t + j * T;
+            if (k >= NK)
+                break;
+            if (k == 0) {
+                C dc = O::fixdiv(O::load(z[0]), 2);
+                gout[0] = O::pack(C{O::add(dc.r, dc.i), O::zero().r});
+                gout[(long long)N * a.out_el] = O::pack(C{O::sub(dc.r, dc.i), O::zero().r});
+                continue;
+            }
+            C xa = O::load(z[k]);
+            C bz = O::load(z[N - k]);
+            C xb{bz.r, O::neg(bz.i)};
+            xa = O::fixdiv(xa, 2);
+            xb = O::fixdiv(xb, 2);
+            C f1 = O::cadd(xa, xb), f2 = O::csub(xa, xb);
+            C tw = O::cmul(f2, O::load(rtw[k - 1]));
+            if (2 * k != N)
+                gout[(long long)k * a.out_el] = O::pack(C{O::half_add(f1.r, tw.r), O::half_add(f1.i, tw.i)});
+            gout[(long long)(N - k) * a.out_el] = O::pack(C{O::half_sub(f1.r, tw.r), O::half_sub(tw.i, f1.i)});
+        }
+    }
+}
+
+}  // namespace kfa

@@ -1,0 +1,29 @@
+// kfa_fast_reg.hpp -- registry of compile-time specialised kernels (see kfa_fast_reg.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "../kfa_engine.h"
+
+namespace kfa {
+
+struct FastArgs;
+
+struct FastVariant {
+    int type;     // KFA_F32 / KFA_Q15 / KFA_Q31
+    int kind;     // FK_C2C / FK_R2C / FK_C2R
+    int n;        // complex line length (the half length for real transforms)
+    int inverse;
+    int T;        // threads per line
+    int LPB;      // lines per workgroup
+    std::vector<int> radices;  // execution order, innermost first
+    int tw_len;   // entries of the per-stage twiddle table
+    const char *name;
+    int (*launch)(const FastArgs &, hipStream_t);
+};
+
+const std::vector<FastVariant> &fast_registry();
+
+}  // namespace kfa

@@ -83,7 +83,7 @@ __device__ __forceinline__ void run_stage(const LinePlan &pl, const StageDesc &s
     typedef Cx<A> C;
     const int p = sd.p, L = cx.L, nb = L / p, Pl = sd.Pl;
     const int nthreads = blockDim.x;
-    if (p <= 5) {
+    if (p >= 2 && p <= 5) {
         const C ya = O::load(tw[(long long)sd.Ptw * sd.mg % pl.N]);
         const C yb = O::load(tw[2LL * sd.Ptw * sd.mg % pl.N]);
         for (int w = threadIdx.x; w < cx.TL * nb; w += nthreads) {
