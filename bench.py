@@ -74,6 +74,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--pmc-run", action="store_true",
+                    help="for rocprofv3 --pmc: one plain copy of a known size (the counter scale), "
+                         "then one forward and one inverse N=4096 launch, and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (batches shard "
@@ -102,6 +105,22 @@ def main() -> int:
             y = torch.empty(BATCH, n, device=dev)
             bufs[n] = (x, X, y, lib.fftr_alloc(n, False), lib.fftr_alloc(n, True))
     stream.synchronize()
+
+    if args.pmc_run:
+        x, X, y, fcfg, icfg = bufs[4096]
+        with torch.cuda.stream(stream):
+            src = torch.empty(256 << 20, dtype=torch.float32, device=dev)  # 1 GiB
+            dst = torch.empty_like(src)
+            dst.copy_(src)
+        lib.fftr_batch(fcfg, x, X, BATCH, 4096, 2049, stream=stream)  # plan + tuning
+        lib.fftri_batch(icfg, X, y, BATCH, 2049, 4096, stream=stream)
+        stream.synchronize()
+        lib.fftr_batch(fcfg, x, X, BATCH, 4096, 2049, stream=stream)
+        lib.fftri_batch(icfg, X, y, BATCH, 2049, 4096, stream=stream)
+        stream.synchronize()
+        print(json.dumps({"pmc_run": True, "copy_bytes": 2 * src.numel() * 4,
+                          "r2c_alg_bytes": alg_bytes(4096) * BATCH}), flush=True)
+        return 0
 
     def step():
         for n in SIZES:

@@ -246,9 +246,10 @@ static bool env_flag(const char *name, bool dflt)
     return !(v[0] == '0' || v[0] == 'n' || v[0] == 'N' || v[0] == 'f' || v[0] == 'F');
 }
 
-// Gathers the stage tables of schedule `rad` from the full-length table: entry (u, r) of
-// stage s is table[r*u*P_s]; then table[P_s*m_s] and table[2*P_s*m_s].
-static int make_stage_table(const DevPlan &P, const std::vector<int> &rad, void **d_out)
+// Gathers the stage tables of schedule `rad` from the full-length table: row u of stage s
+// holds table[r*u*P_s] for r = 1..p-1, padded to a multiple of `pad` entries; then
+// table[P_s*m_s] and table[2*P_s*m_s], padded likewise (the layout of Radices<G, ...>).
+static int make_stage_table(const DevPlan &P, const std::vector<int> &rad, int pad, void **d_out)
 {
     const size_t cb = cpx_bytes(P.type);
     const long long n = P.n;
@@ -258,13 +259,24 @@ static int make_stage_table(const DevPlan &P, const std::vector<int> &rad, void 
         const unsigned char *src = P.h_tw.data() + cb * (size_t)(idx % n);
         h.insert(h.end(), src, src + cb);
     };
+    auto fill_to = [&](size_t entries) {
+        while (h.size() < entries * cb)
+            put(0);
+    };
+    auto rup = [](int v, int g) { return (v + g - 1) / g * g; };
     for (int p : rad) {
         const long long Pst = n / (p * m);
-        for (long long u = 0; u < m; ++u)
+        const int PR = rup(p - 1 > 0 ? p - 1 : 1, pad);
+        for (long long u = 0; u < m; ++u) {
+            const size_t row0 = h.size() / cb;
             for (int r = 1; r < p; ++r)
                 put((long long)r * u * Pst);
+            fill_to(row0 + PR);
+        }
+        const size_t c0 = h.size() / cb;
         put(Pst * m);
         put(2 * Pst * m);
+        fill_to(c0 + rup(2, pad));
         m *= p;
     }
     KFA_HIP(hipMalloc(d_out, h.size()));
@@ -295,15 +307,29 @@ static int choose_fast(DevPlan &P, int kind, hipStream_t s, const FastChoice **o
         return 0;
     std::vector<void *> tabs(cand.size(), nullptr);
     for (size_t i = 0; i < cand.size(); ++i) {
-        int rc = make_stage_table(P, cand[i]->radices, &tabs[i]);
+        if ((long long)(cand[i]->tw_len) <= 0)
+            return fail(-2, "bad fast variant table");
+        int rc = make_stage_table(P, cand[i]->radices, cand[i]->tw_pad, &tabs[i]);
         if (rc)
             return rc;
     }
     size_t best = 0;
-    if (cand.size() > 1 && env_flag("KISSFFT_AMD_TUNE", true)) {
-        // time every candidate on a scratch batch of this length
-        const long long lines = P.n >= 2048 ? 8192 : 32768;
+    const char *force = getenv("KISSFFT_AMD_VARIANT");
+    bool forced = false;
+    if (force && *force) {
+        for (size_t i = 0; i < cand.size(); ++i)
+            if (strcmp(force, cand[i]->name) == 0) {
+                best = i;
+                forced = true;
+            }
+    }
+    if (!forced && cand.size() > 1 && env_flag("KISSFFT_AMD_TUNE", true)) {
+        // time every candidate on a scratch batch large enough to stream from HBM
+        // (about 512 MiB in + out, beyond the last-level cache)
         const size_t cb = cpx_bytes(P.type);
+        long long lines = (256LL << 20) / ((long long)cb * (P.n + 1));
+        if (lines < 64)
+            lines = 64;
         const size_t bytes = cb * (size_t)(P.n + 1) * (size_t)lines * 2;
         void *bin = nullptr, *bout = nullptr, *brtw = nullptr;
         KFA_HIP(hipMallocAsync(&bin, bytes, s));
@@ -362,10 +388,13 @@ static int choose_fast(DevPlan &P, int kind, hipStream_t s, const FastChoice **o
 // ------------------------------------------------------------------------------------
 static thread_local hipStream_t t_stream = nullptr;
 
+// The calling thread's stream: a blocking stream, so synchronous calls are ordered after
+// work the caller queued on the legacy default stream (e.g. whatever produced a device
+// input) and before work queued there afterwards.  Default priority.
 static int thread_stream(hipStream_t *s)
 {
     if (!t_stream)
-        KFA_HIP(hipStreamCreateWithFlags(&t_stream, hipStreamNonBlocking));
+        KFA_HIP(hipStreamCreateWithFlags(&t_stream, hipStreamDefault));
     *s = t_stream;
     return 0;
 }

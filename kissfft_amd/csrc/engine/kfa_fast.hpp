@@ -26,7 +26,12 @@ namespace kfa {
 
 enum { FK_C2C = 0, FK_R2C = 1, FK_C2R = 2 };
 
-template <int... R>
+// Stage-table layout: stage s holds m_s rows of PR(p_s) entries (entry r-1 of row u is the
+// twiddle of input r at column u; rows padded to G entries so a row is a 16-byte aligned
+// run), then the two fixed rotations padded to G entries.
+constexpr int round_up(int v, int g) { return (v + g - 1) / g * g; }
+
+template <int G, int... R>
 struct Radices {
     static constexpr int K = sizeof...(R);
     static constexpr int r[K] = {R...};
@@ -39,11 +44,12 @@ struct Radices {
         return v;
     }
     static constexpr int N() { return m(K); }
+    static constexpr int PR(int s) { return round_up(r[s] - 1 > 0 ? r[s] - 1 : 1, G); }
     static constexpr int tw_off(int s)
     {
         int o = 0;
         for (int i = 0; i < s; ++i)
-            o += m(i) * (r[i] - 1) + 2;
+            o += m(i) * PR(i) + round_up(2, G);
         return o;
     }
     static constexpr int tw_len() { return tw_off(K); }
@@ -163,102 +169,181 @@ struct Dft<8, INV> {
     }
 };
 
+// multiply by the constant (c + i*s)
+__device__ __forceinline__ Cf cf_mulc(Cf a, float c, float s) { return Cf{a.r * c - a.i * s, a.r * s + a.i * c}; }
+
+// 16 = 4 x 4: column DFT4s, twiddles W16^(n2*k1), row DFT4s.
+template <bool INV>
+struct Dft<16, INV> {
+    __device__ __forceinline__ static void run(Cf *v)
+    {
+        const float c1 = 0.923879532511286756128183189396788933f;  // cos(pi/8)
+        const float s1 = 0.382683432365089771728459984030398867f;  // sin(pi/8)
+        const float h = 0.707106781186547524400844362104849039f;
+        const float sg = INV ? 1.f : -1.f;  // sign of the exponent
+        Cf a[4][4];
+#pragma unroll
+        for (int n2 = 0; n2 < 4; ++n2) {
+            Cf e[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
+            Dft<4, INV>::run(e);
+#pragma unroll
+            for (int k1 = 0; k1 < 4; ++k1)
+                a[k1][n2] = e[k1];
+        }
+        // W16^j = cos(2 pi j/16) + sg*i*sin(2 pi j/16) for j = n2*k1 in {1,2,3,4,6,9}
+        a[1][1] = cf_mulc(a[1][1], c1, sg * s1);
+        a[1][2] = cf_mulc(a[1][2], h, sg * h);
+        a[1][3] = cf_mulc(a[1][3], s1, sg * c1);
+        a[2][1] = cf_mulc(a[2][1], h, sg * h);
+        a[2][2] = cf_rot<INV>(a[2][2]);
+        a[2][3] = cf_mulc(a[2][3], -h, sg * h);
+        a[3][1] = cf_mulc(a[3][1], s1, sg * c1);
+        a[3][2] = cf_mulc(a[3][2], -h, sg * h);
+        a[3][3] = cf_mulc(a[3][3], -c1, -sg * s1);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            Dft<4, INV>::run(a[k1]);
+#pragma unroll
+            for (int k2 = 0; k2 < 4; ++k2)
+                v[k1 + 4 * k2] = a[k1][k2];
+        }
+    }
+};
+
 // ------------------------------------------------------------------------------------
 // one stage
 // ------------------------------------------------------------------------------------
-template <class A, bool INV, int N, int T, int p, int m, int TWO, bool GSRC, bool GDST>
-__device__ __forceinline__ void fast_stage(const typename A::store2 *__restrict__ src, long long src_el,
-                                           typename A::store2 *__restrict__ dst, long long dst_el,
-                                           const typename A::store2 *__restrict__ stw, int t, bool valid)
-{
+// Stage s of schedule RS for one thread: gather its J butterflies' inputs (from HBM for
+// the first stage, else from LDS), then twiddle, transform and scatter the outputs (to
+// HBM for the last stage, else to LDS).  With a single LDS buffer a barrier separates
+// the gather from the scatter.
+template <class A, bool INV, int T, class RS, int S>
+struct FastStage {
+    typedef typename A::store2 St;
     typedef Ops<A> O;
     typedef Cx<A> C;
-    constexpr int NB = N / p;
-    constexpr int P = N / (p * m);
-    constexpr int J = (NB + T - 1) / T;
+    static constexpr int N = RS::N();
+    static constexpr int p = RS::p(S);
+    static constexpr int m = RS::m(S);
+    static constexpr int NB = N / p;
+    static constexpr int P = N / (p * m);
+    static constexpr int J = (NB + T - 1) / T;
+    static constexpr int PR = RS::PR(S);
+    static constexpr int TWO = RS::tw_off(S);
+
+    __device__ __forceinline__ static bool live(int t, int j) { return (NB % T) == 0 || t + j * T < NB; }
+
+    template <bool GSRC>
+    __device__ __forceinline__ static void gather(C (&v)[J][p], const St *__restrict__ src, long long src_el, int t,
+                                                  bool valid)
+    {
 #pragma unroll
-    for (int j = 0; j < J; ++j) {
-        const int b = t + j * T;
-        if ((NB % T) != 0 && b >= NB)
-            break;
-        const int u = b / P, o = b - (b / P) * P;
-        C v[p];
+        for (int j = 0; j < J; ++j) {
+            if (!live(t, j))
+                break;
+            const int b = t + j * T;
+            const int u = b / P, o = b - (b / P) * P;
 #pragma unroll
-        for (int r = 0; r < p; ++r) {
-            const int idx = u * P * p + r * P + o;
-            if (GSRC)
-                v[r] = valid ? O::load(src[idx * src_el]) : O::zero();
-            else
-                v[r] = O::load(src[idx]);
-        }
-        if constexpr (A::fixed) {
-            C w[p > 1 ? p - 1 : 1];
-#pragma unroll
-            for (int r = 1; r < p; ++r)
-                w[r - 1] = O::load(stw[TWO + u * (p - 1) + r - 1]);
-            const C ya = O::load(stw[TWO + m * (p - 1)]);
-            const C yb = O::load(stw[TWO + m * (p - 1) + 1]);
-            if constexpr (p == 2)
-                bfly2<A>(v, w);
-            else if constexpr (p == 3)
-                bfly3<A>(v, w, ya);
-            else if constexpr (p == 4)
-                bfly4<A>(v, w, INV);
-            else
-                bfly5<A>(v, w, ya, yb);
-            (void)yb;
-        } else {
-            if constexpr (m > 1) {
-#pragma unroll
-                for (int r = 1; r < p; ++r)
-                    v[r] = O::cmul(v[r], O::load(stw[TWO + u * (p - 1) + r - 1]));
-            }
-            Dft<p, INV>::run(v);
-        }
-#pragma unroll
-        for (int q = 0; q < p; ++q) {
-            const int idx = b + q * NB;
-            if (GDST) {
-                if (valid)
-                    dst[idx * dst_el] = O::pack(v[q]);
-            } else {
-                dst[idx] = O::pack(v[q]);
+            for (int r = 0; r < p; ++r) {
+                const int idx = u * P * p + r * P + o;
+                if (GSRC)
+                    v[j][r] = valid ? O::load(src[idx * src_el]) : O::zero();
+                else
+                    v[j][r] = O::load(src[idx]);
             }
         }
     }
-}
 
-template <class A, bool INV, int T, int KIND, class RS, int S>
+    template <bool GDST>
+    __device__ __forceinline__ static void scatter(C (&v)[J][p], St *__restrict__ dst, long long dst_el,
+                                                   const St *__restrict__ stw, int t, bool valid)
+    {
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            if (!live(t, j))
+                break;
+            const int b = t + j * T;
+            const int u = b / P;
+            const St *row = stw + TWO + u * PR;
+            if constexpr (A::fixed) {
+                C w[p > 1 ? p - 1 : 1];
+#pragma unroll
+                for (int r = 1; r < p; ++r)
+                    w[r - 1] = O::load(row[r - 1]);
+                const C ya = O::load(stw[TWO + m * PR]);
+                const C yb = O::load(stw[TWO + m * PR + 1]);
+                if constexpr (p == 2)
+                    bfly2<A>(v[j], w);
+                else if constexpr (p == 3)
+                    bfly3<A>(v[j], w, ya);
+                else if constexpr (p == 4)
+                    bfly4<A>(v[j], w, INV);
+                else
+                    bfly5<A>(v[j], w, ya, yb);
+                (void)yb;
+            } else {
+                if constexpr (m > 1) {
+#pragma unroll
+                    for (int r = 1; r < p; ++r)
+                        v[j][r] = O::cmul(v[j][r], O::load(row[r - 1]));
+                }
+                Dft<p, INV>::run(v[j]);
+            }
+#pragma unroll
+            for (int q = 0; q < p; ++q) {
+                const int idx = b + q * NB;
+                if (GDST) {
+                    if (valid)
+                        dst[idx * dst_el] = O::pack(v[j][q]);
+                } else {
+                    dst[idx] = O::pack(v[j][q]);
+                }
+            }
+        }
+    }
+};
+
+// Runs stage S.  Double-buffered LDS (DB): stage S reads buffer S%2 and writes the other,
+// one barrier per stage.  Single buffer: gather, barrier, scatter, barrier.
+template <class A, bool INV, int T, int KIND, bool DB, class RS, int S>
 __device__ __forceinline__ void fast_run_one(typename A::store2 *bufA, typename A::store2 *bufB,
                                              const typename A::store2 *gin, long long in_el,
                                              typename A::store2 *gout, long long out_el,
                                              const typename A::store2 *stw, int t, bool valid)
 {
     typedef typename A::store2 Sx;
+    typedef FastStage<A, INV, T, RS, S> F;
     constexpr int K = RS::K;
-    static_assert(K >= 1, "empty schedule");
-    constexpr int N = RS::N();
     constexpr bool GSRC = (S == 0) && (KIND != FK_C2R);
     constexpr bool GDST = (S == K - 1) && (KIND != FK_R2C);
-    const Sx *src = GSRC ? gin : ((S % 2 == 0) ? bufA : bufB);
-    Sx *dst = GDST ? gout : ((S % 2 == 0) ? bufB : bufA);
-    fast_stage<A, INV, N, T, RS::p(S), RS::m(S), RS::tw_off(S), GSRC, GDST>(src, GSRC ? in_el : 1, dst,
-                                                                           GDST ? out_el : 1, stw, t, valid);
+    const Sx *src = GSRC ? gin : ((!DB || S % 2 == 0) ? bufA : bufB);
+    Sx *dst = GDST ? gout : ((!DB || S % 2 == 1) ? bufA : bufB);
+    typename F::C v[F::J][F::p];
+    F::template gather<GSRC>(v, src, GSRC ? in_el : 1, t, valid);
+    if constexpr (!DB && !GSRC && !GDST)
+        __syncthreads();  // every input read before the buffer is overwritten
+    F::template scatter<GDST>(v, dst, GDST ? out_el : 1, stw, t, valid);
     if constexpr (!GDST)
         __syncthreads();
 }
 
-template <class A, bool INV, int T, int KIND, class RS, int... S>
+template <class A, bool INV, int T, int KIND, bool DB, class RS, int... S>
 __device__ __forceinline__ void fast_run_all(std::integer_sequence<int, S...>, typename A::store2 *bufA,
                                              typename A::store2 *bufB, const typename A::store2 *gin,
                                              long long in_el, typename A::store2 *gout, long long out_el,
                                              const typename A::store2 *stw, int t, bool valid)
 {
-    (fast_run_one<A, INV, T, KIND, RS, S>(bufA, bufB, gin, in_el, gout, out_el, stw, t, valid), ...);
+    (fast_run_one<A, INV, T, KIND, DB, RS, S>(bufA, bufB, gin, in_el, gout, out_el, stw, t, valid), ...);
 }
 
-// Dynamic LDS: 2 * LPB * N complex values.
-template <class A, int KIND, bool INV, int T, int LPB, class RS>
+// buffer holding the result of the last stage (input of the real split)
+template <bool DB, int K>
+struct LastBuf {
+    static constexpr bool is_a = !DB || (K % 2 == 0);
+};
+
+// Dynamic LDS: (DB ? 2 : 1) * LPB * N complex values.
+template <class A, int KIND, bool INV, int T, int LPB, bool DB, class RS>
 __global__ void __launch_bounds__(T *LPB) k_fast(FastArgs a)
 {
     typedef typename A::store2 S;
@@ -272,10 +357,10 @@ __global__ void __launch_bounds__(T *LPB) k_fast(FastArgs a)
     const long long gl = (long long)blockIdx.x * LPB + line;
     const bool valid = gl < a.nlines;
     S *bufA = reinterpret_cast<S *>(kfa_fast_smem) + line * N;
-    S *bufB = reinterpret_cast<S *>(kfa_fast_smem) + (LPB + line) * N;
+    S *bufB = DB ? reinterpret_cast<S *>(kfa_fast_smem) + (LPB + line) * N : bufA;
     const S *gin = reinterpret_cast<const S *>(a.in) + (valid ? gl * a.in_hi : 0);
     S *gout = reinterpret_cast<S *>(a.out) + (valid ? gl * a.out_hi : 0);
-    const S *stw = reinterpret_cast<const S *>(a.stw);
+    const S *stw = reinterpret_cast<const S *>(__builtin_assume_aligned(a.stw, 16));
     const S *rtw = reinterpret_cast<const S *>(a.rtw);
 
     if constexpr (KIND == FK_C2R) {
@@ -310,12 +395,12 @@ __global__ void __launch_bounds__(T *LPB) k_fast(FastArgs a)
         __syncthreads();
     }
 
-    fast_run_all<A, INV, T, KIND, RS>(std::make_integer_sequence<int, K>{}, bufA, bufB, gin, a.in_el, gout,
-                                      a.out_el, stw, t, valid);
+    fast_run_all<A, INV, T, KIND, DB, RS>(std::make_integer_sequence<int, K>{}, bufA, bufB, gin, a.in_el, gout,
+                                          a.out_el, stw, t, valid);
 
     if constexpr (KIND == FK_R2C) {
         // split (kiss_fftr.c:88-116) from the LDS buffer the last stage wrote
-        const S *z = (K % 2 == 0) ? bufA : bufB;
+        const S *z = LastBuf<DB, K>::is_a ? bufA : bufB;
         constexpr int NK = N / 2 + 1;
         constexpr int JK = (NK + T - 1) / T;
         if (!valid)

@@ -1,10 +1,11 @@
 // kfa_fast_reg.hip -- the instantiated fast-kernel variants and their launchers.
 //
 // Each variant is (scalar kind, transform kind, line length, direction, radix schedule,
-// threads per line, lines per workgroup).  Fixed-point variants carry the reference radix
-// schedule of their length (kf_factor order, executed innermost first) and are used only
-// when the plan's schedule equals it; float variants may use any schedule.  When several
-// variants serve the same key the engine times them on first use and keeps the fastest.
+// threads per line, lines per workgroup, LDS buffering).  Fixed-point variants carry the
+// reference radix schedule of their length (kf_factor order, executed innermost first) and
+// are used only when the plan's schedule equals it; float variants may use any schedule.
+// When several variants serve the same key the engine times them on first use and keeps
+// the fastest.
 #include <hip/hip_runtime.h>
 
 #include <mutex>
@@ -14,28 +15,36 @@
 
 namespace kfa {
 
-template <class A, int KIND, bool INV, int T, int LPB, class RS>
+template <class A, int KIND, bool INV, int T, int LPB, bool DB, class RS>
 static int launch_fast(const FastArgs &a, hipStream_t s)
 {
     typedef typename A::store2 S;
-    const size_t lds = 2 * (size_t)LPB * RS::N() * sizeof(S);
+    const size_t lds = (DB ? 2 : 1) * (size_t)LPB * RS::N() * sizeof(S);
     static std::once_flag once;
     std::call_once(once, [=] {
-        (void)hipFuncSetAttribute((const void *)k_fast<A, KIND, INV, T, LPB, RS>,
+        (void)hipFuncSetAttribute((const void *)k_fast<A, KIND, INV, T, LPB, DB, RS>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     });
     const long long blocks = (a.nlines + LPB - 1) / LPB;
     if (blocks <= 0)
         return 0;
-    hipLaunchKernelGGL((k_fast<A, KIND, INV, T, LPB, RS>), dim3((unsigned)blocks), dim3(T * LPB), lds, s, a);
+    hipLaunchKernelGGL((k_fast<A, KIND, INV, T, LPB, DB, RS>), dim3((unsigned)blocks), dim3(T * LPB), lds, s, a);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-template <class A, int KIND, int T, int LPB, int... R>
+// twiddle-row granularity: 16-byte aligned rows
+template <class A>
+struct TwPad {
+    static constexpr int G = 16 / (int)sizeof(typename A::store2);
+};
+
+template <class A, int KIND, int T, int LPB, bool DB, int... R>
 static void add_variant(std::vector<FastVariant> &v, int type, const char *name)
 {
-    typedef Radices<R...> RS;
+    constexpr int G = TwPad<A>::G;
+    typedef Radices<G, R...> RS;
     static_assert(RS::K >= 2, "single-stage schedules read and write HBM in one stage (not in-place safe)");
+    static_assert(T * LPB <= 1024, "workgroup too large");
     for (int inv = 0; inv < 2; ++inv) {
         if (KIND == FK_R2C && inv)
             continue;
@@ -49,45 +58,65 @@ static void add_variant(std::vector<FastVariant> &v, int type, const char *name)
         f.T = T;
         f.LPB = LPB;
         f.radices = {R...};
+        f.tw_pad = G;
         f.tw_len = RS::tw_len();
         f.name = name;
-        f.launch = inv ? &launch_fast<A, KIND, true, T, LPB, RS> : &launch_fast<A, KIND, false, T, LPB, RS>;
+        f.launch = inv ? &launch_fast<A, KIND, true, T, LPB, DB, RS> : &launch_fast<A, KIND, false, T, LPB, DB, RS>;
         v.push_back(f);
     }
+}
+
+// forward and inverse real variants of one schedule
+template <class A, int T, int LPB, bool DB, int... R>
+static void add_real(std::vector<FastVariant> &v, int type, const char *name)
+{
+    add_variant<A, FK_R2C, T, LPB, DB, R...>(v, type, name);
+    add_variant<A, FK_C2R, T, LPB, DB, R...>(v, type, name);
 }
 
 const std::vector<FastVariant> &fast_registry()
 {
     static std::vector<FastVariant> *reg = [] {
         auto *v = new std::vector<FastVariant>();
-        // ---- float, real transforms: half lengths of the real workloads ----
-        add_variant<F32, FK_R2C, 256, 1, 8, 8, 8, 4>(*v, KFA_F32, "r2c2048_8884_t256");
-        add_variant<F32, FK_R2C, 256, 1, 4, 8, 8, 8>(*v, KFA_F32, "r2c2048_4888_t256");
-        add_variant<F32, FK_R2C, 128, 2, 8, 8, 8, 4>(*v, KFA_F32, "r2c2048_8884_t128");
-        add_variant<F32, FK_C2R, 256, 1, 8, 8, 8, 4>(*v, KFA_F32, "c2r2048_8884_t256");
-        add_variant<F32, FK_C2R, 256, 1, 4, 8, 8, 8>(*v, KFA_F32, "c2r2048_4888_t256");
-        add_variant<F32, FK_C2R, 128, 2, 8, 8, 8, 4>(*v, KFA_F32, "c2r2048_8884_t128");
-        add_variant<F32, FK_R2C, 100, 2, 4, 4, 5, 5, 5>(*v, KFA_F32, "r2c2000_44555_t100");
-        add_variant<F32, FK_R2C, 200, 1, 5, 5, 5, 4, 4>(*v, KFA_F32, "r2c2000_55544_t200");
-        add_variant<F32, FK_C2R, 100, 2, 4, 4, 5, 5, 5>(*v, KFA_F32, "c2r2000_44555_t100");
-        add_variant<F32, FK_C2R, 200, 1, 5, 5, 5, 4, 4>(*v, KFA_F32, "c2r2000_55544_t200");
-        add_variant<F32, FK_R2C, 50, 4, 4, 5, 5, 5>(*v, KFA_F32, "r2c500_4555_t50");
-        add_variant<F32, FK_C2R, 50, 4, 4, 5, 5, 5>(*v, KFA_F32, "c2r500_4555_t50");
-        // ---- float, complex ----
-        add_variant<F32, FK_C2C, 240, 1, 4, 4, 3, 5, 5>(*v, KFA_F32, "c2c1200_44355_t240");
-        add_variant<F32, FK_C2C, 120, 2, 5, 5, 3, 4, 4>(*v, KFA_F32, "c2c1200_55344_t120");
-        add_variant<F32, FK_C2C, 256, 1, 8, 8, 8, 4>(*v, KFA_F32, "c2c2048_8884_t256");
-        add_variant<F32, FK_C2C, 128, 2, 8, 8, 4, 4>(*v, KFA_F32, "c2c1024_8844_t128");
-        add_variant<F32, FK_C2C, 64, 4, 8, 8, 8>(*v, KFA_F32, "c2c512_888_t64");
-        add_variant<F32, FK_C2C, 64, 4, 4, 4, 4, 4>(*v, KFA_F32, "c2c256_4444_t64");
-        add_variant<F32, FK_C2C, 100, 2, 5, 5, 5, 4, 2>(*v, KFA_F32, "c2c1000_55542_t100");
+        // ---- float real transforms; the line is the half length ----
+        // N = 4096 (half 2048)
+        add_real<F32, 256, 1, true, 8, 8, 8, 4>(*v, KFA_F32, "real2048_8884_t256_db");
+        add_real<F32, 256, 1, false, 8, 8, 8, 4>(*v, KFA_F32, "real2048_8884_t256_sb");
+        add_real<F32, 128, 1, false, 16, 16, 8>(*v, KFA_F32, "real2048_16168_t128_sb");
+        add_real<F32, 128, 1, true, 16, 16, 8>(*v, KFA_F32, "real2048_16168_t128_db");
+        add_real<F32, 128, 1, false, 8, 16, 16>(*v, KFA_F32, "real2048_81616_t128_sb");
+        add_real<F32, 128, 2, false, 16, 16, 8>(*v, KFA_F32, "real2048_16168_t128x2_sb");
+        // N = 4000 (half 2000)
+        add_real<F32, 100, 2, true, 4, 4, 5, 5, 5>(*v, KFA_F32, "real2000_44555_t100x2_db");
+        add_real<F32, 100, 2, false, 4, 4, 5, 5, 5>(*v, KFA_F32, "real2000_44555_t100x2_sb");
+        add_real<F32, 250, 1, false, 5, 5, 5, 4, 4>(*v, KFA_F32, "real2000_55544_t250_sb");
+        add_real<F32, 125, 2, false, 16, 5, 5, 5>(*v, KFA_F32, "real2000_16555_t125x2_sb");
+        add_real<F32, 250, 1, false, 8, 2, 5, 5, 5>(*v, KFA_F32, "real2000_82555_t250_sb");
+        // N = 1000 (half 500), the real axis of the 2-D workload
+        add_real<F32, 50, 4, false, 4, 5, 5, 5>(*v, KFA_F32, "real500_4555_t50x4_sb");
+        add_real<F32, 125, 2, false, 4, 5, 5, 5>(*v, KFA_F32, "real500_4555_t125x2_sb");
+        // ---- float complex ----
+        add_variant<F32, FK_C2C, 240, 1, false, 4, 4, 3, 5, 5>(*v, KFA_F32, "c2c1200_44355_t240_sb");
+        add_variant<F32, FK_C2C, 240, 1, true, 4, 4, 3, 5, 5>(*v, KFA_F32, "c2c1200_44355_t240_db");
+        add_variant<F32, FK_C2C, 120, 2, false, 5, 5, 3, 16>(*v, KFA_F32, "c2c1200_55316_t120x2_sb");
+        add_variant<F32, FK_C2C, 128, 1, false, 16, 16, 8>(*v, KFA_F32, "c2c2048_16168_t128_sb");
+        add_variant<F32, FK_C2C, 256, 1, false, 8, 8, 8, 4>(*v, KFA_F32, "c2c2048_8884_t256_sb");
+        add_variant<F32, FK_C2C, 64, 2, false, 16, 16, 4>(*v, KFA_F32, "c2c1024_16164_t64x2_sb");
+        add_variant<F32, FK_C2C, 128, 2, false, 8, 8, 4, 4>(*v, KFA_F32, "c2c1024_8844_t128x2_sb");
+        add_variant<F32, FK_C2C, 64, 4, false, 8, 8, 8>(*v, KFA_F32, "c2c512_888_t64x4_sb");
+        add_variant<F32, FK_C2C, 16, 16, false, 16, 16>(*v, KFA_F32, "c2c256_1616_t16x16_sb");
+        add_variant<F32, FK_C2C, 64, 4, false, 4, 4, 4, 4>(*v, KFA_F32, "c2c256_4444_t64x4_sb");
+        add_variant<F32, FK_C2C, 100, 2, false, 5, 5, 5, 4, 2>(*v, KFA_F32, "c2c1000_55542_t100x2_sb");
+        add_variant<F32, FK_C2C, 125, 2, false, 5, 5, 5, 8>(*v, KFA_F32, "c2c1000_5558_t125x2_sb");
+        add_variant<F32, FK_C2C, 192, 1, false, 3, 16, 8, 8>(*v, KFA_F32, "c2c3072_31688_t192_sb");
+        add_variant<F32, FK_C2C, 256, 1, false, 16, 16, 16>(*v, KFA_F32, "c2c4096_161616_t256_sb");
         // ---- Q15 (reference schedules) ----
-        add_variant<Q15, FK_C2C, 128, 2, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15c2c512_t128");
-        add_variant<Q15, FK_C2C, 64, 4, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15c2c512_t64");
-        add_variant<Q15, FK_R2C, 128, 2, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15r2c1024_t128");
-        add_variant<Q15, FK_C2R, 128, 2, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15c2r1024_t128");
-        add_variant<Q15, FK_R2C, 256, 1, 2, 4, 4, 4, 4, 4>(*v, KFA_Q15, "q15r2c4096_t256");
-        add_variant<Q15, FK_C2R, 256, 1, 2, 4, 4, 4, 4, 4>(*v, KFA_Q15, "q15c2r4096_t256");
+        add_variant<Q15, FK_C2C, 128, 2, true, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15c2c512_t128x2_db");
+        add_variant<Q15, FK_C2C, 128, 2, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15c2c512_t128x2_sb");
+        add_variant<Q15, FK_C2C, 64, 4, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15c2c512_t64x4_sb");
+        add_variant<Q15, FK_C2C, 128, 4, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15c2c512_t128x4_sb");
+        add_real<Q15, 128, 2, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15real512_t128x2_sb");
+        add_real<Q15, 256, 1, false, 2, 4, 4, 4, 4, 4>(*v, KFA_Q15, "q15real2048_t256_sb");
         return v;
     }();
     return *reg;

@@ -19,6 +19,7 @@ struct FastVariant {
     int T;        // threads per line
     int LPB;      // lines per workgroup
     std::vector<int> radices;  // execution order, innermost first
+    int tw_pad;   // stage-table rows padded to a multiple of this many entries
     int tw_len;   // entries of the per-stage twiddle table
     const char *name;
     int (*launch)(const FastArgs &, hipStream_t);

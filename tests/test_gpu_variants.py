@@ -1,0 +1,103 @@
+"""Every registered fast-kernel variant, forced one at a time, against the oracle / FP64.
+
+The engine times the variants of a size on first use and keeps the fastest; this test
+does not rely on that choice: it forces each variant by name (KISSFFT_AMD_VARIANT),
+drops the plan cache between variants (kiss_fft_cleanup), and checks the result --
+bit-exact for Q15, rel-L2 for float.  The variant list is read from the registry source.
+"""
+import os
+import re
+import sys
+
+import numpy as np
+import pytest
+
+import kissfft_amd
+from conftest import ROOT
+from helpers import ABS_FLOAT_BOUND, cplx, rel_l2
+from kissfft_amd import workloads as W
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+REG = os.path.join(ROOT, "kissfft_amd", "csrc", "engine", "kfa_fast_reg.hip")
+
+
+def registry():
+    src = open(REG).read()
+    out = []
+    for m in re.finditer(r"add_(real|variant)<(\w+),\s*(?:(FK_\w+),\s*)?(\d+),\s*(\d+),\s*(true|false),\s*([\d,\s]+)>"
+                         r"\(\*v,\s*KFA_\w+,\s*\"(\w+)\"\)", src):
+        kind_fn, typ, fk, T, LPB, db, rad, name = m.groups()
+        radices = [int(r) for r in rad.replace(" ", "").split(",") if r]
+        n = int(np.prod(radices))
+        kinds = ["r2c", "c2r"] if kind_fn == "real" else [{"FK_C2C": "c2c", "FK_R2C": "r2c", "FK_C2R": "c2r"}[fk]]
+        for k in kinds:
+            out.append((typ, k, n, name))
+    return out
+
+
+CASES = registry()
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_registry_parsed():
+    assert len(CASES) >= 20
+
+
+@pytest.mark.parametrize("typ,kind,n,name", CASES, ids=[f"{c[3]}-{c[1]}" for c in CASES])
+def test_forced_variant(typ, kind, n, name, monkeypatch):
+    datatype = {"F32": "float", "Q15": "int16_t"}[typ]
+    lib = kissfft_amd.load(datatype)
+    lib.L.kiss_fft_cleanup()
+    monkeypatch.setenv("KISSFFT_AMD_VARIANT", name)
+    say("variant", name, kind, n)
+    rng = np.random.default_rng(hash(name) & 0xFFFF)
+    B = 37  # not a multiple of any lines-per-workgroup: exercises the partial last workgroup
+    try:
+        if datatype == "int16_t":
+            O = Oracle("int16_t")
+            if kind == "c2c":
+                x = rng.integers(-32768, 32768, (B, n, 2)).astype(np.int16)
+                for inv in (False, True):
+                    cfg = lib.fft_alloc(n, inv)
+                    y = np.zeros_like(x)
+                    lib.fft_batch(cfg, x, y, B, n, n)
+                    np.testing.assert_array_equal(y, O.fft(x, inv))
+            elif kind == "r2c":
+                x = rng.integers(-32768, 32768, (B, 2 * n)).astype(np.int16)
+                y = np.zeros((B, n + 1, 2), np.int16)
+                lib.fftr_batch(lib.fftr_alloc(2 * n), x, y, B, 2 * n, n + 1)
+                np.testing.assert_array_equal(y, O.fftr(x))
+            else:
+                X = rng.integers(-32768, 32768, (B, n + 1, 2)).astype(np.int16)
+                y = np.zeros((B, 2 * n), np.int16)
+                lib.fftri_batch(lib.fftr_alloc(2 * n, True), X, y, B, n + 1, 2 * n)
+                np.testing.assert_array_equal(y, O.fftri(X, 2 * n))
+        else:
+            if kind == "c2c":
+                x = rng.uniform(-1, 1, (B, n, 2)).astype(np.float32)
+                for inv in (False, True):
+                    cfg = lib.fft_alloc(n, inv)
+                    y = np.zeros_like(x)
+                    lib.fft_batch(cfg, x, y, B, n, n)
+                    want = np.fft.ifft(cplx(x), axis=-1) * n if inv else np.fft.fft(cplx(x), axis=-1)
+                    assert rel_l2(cplx(y), want) <= ABS_FLOAT_BOUND
+            elif kind == "r2c":
+                x = W.two_tone(7, 2 * n, B)
+                y = np.zeros((B, n + 1, 2), np.float32)
+                lib.fftr_batch(lib.fftr_alloc(2 * n), x, y, B, 2 * n, n + 1)
+                assert rel_l2(cplx(y), np.fft.rfft(x.astype(np.float64), axis=-1)) <= 3e-7
+            else:
+                Xc = np.fft.rfft(W.two_tone(8, 2 * n, B).astype(np.float64), axis=-1).astype(np.complex64)
+                X = np.stack([Xc.real, Xc.imag], -1).astype(np.float32)
+                y = np.zeros((B, 2 * n), np.float32)
+                lib.fftri_batch(lib.fftr_alloc(2 * n, True), X, y, B, n + 1, 2 * n)
+                want = np.fft.irfft(Xc.astype(np.complex128), n=2 * n, axis=-1) * 2 * n
+                assert rel_l2(y, want) <= 4e-7
+    finally:
+        lib.L.kiss_fft_cleanup()
