@@ -213,11 +213,48 @@ struct Dft<16, INV> {
 // ------------------------------------------------------------------------------------
 // one stage
 // ------------------------------------------------------------------------------------
-// Stage s of schedule RS for one thread: gather its J butterflies' inputs (from HBM for
+// ------------------------------------------------------------------------------------
+// Real split / merge of one bin pair (k, N-k), 1 <= k <= N/2 (kiss_fftr.c:101-116 and
+// kiss_fftr.c:135-153).  For k == N/2 the pair is one bin and the second store wins, as in
+// the reference loops.
+// ------------------------------------------------------------------------------------
+template <class A>
+__device__ __forceinline__ void split_pair(Cx<A> zk, Cx<A> znk, Cx<A> w, Cx<A> &outk, Cx<A> &outnk)
+{
+    typedef Ops<A> O;
+    typedef Cx<A> C;
+    C a = O::fixdiv(zk, 2);
+    C b = O::fixdiv(C{znk.r, O::neg(znk.i)}, 2);
+    C f1 = O::cadd(a, b), f2 = O::csub(a, b);
+    C tw = O::cmul(f2, w);
+    outk = C{O::half_add(f1.r, tw.r), O::half_add(f1.i, tw.i)};
+    outnk = C{O::half_sub(f1.r, tw.r), O::half_sub(tw.i, f1.i)};
+}
+
+template <class A>
+__device__ __forceinline__ void merge_pair(Cx<A> xk, Cx<A> xnk, Cx<A> w, Cx<A> &zk, Cx<A> &znk)
+{
+    typedef Ops<A> O;
+    typedef Cx<A> C;
+    C a = O::fixdiv(xk, 2);
+    C b = O::fixdiv(C{xnk.r, O::neg(xnk.i)}, 2);
+    C e = O::cadd(a, b), d = O::csub(a, b);
+    C o = O::cmul(d, w);
+    zk = O::cadd(e, o);
+    C lo = O::csub(e, o);
+    znk = C{lo.r, O::neg(lo.i)};
+}
+
+// Stage S of schedule RS for one thread: gather its J butterflies' inputs (from HBM for
 // the first stage, else from LDS), then twiddle, transform and scatter the outputs (to
 // HBM for the last stage, else to LDS).  With a single LDS buffer a barrier separates
 // the gather from the scatter.
-template <class A, bool INV, int T, class RS, int S>
+//
+// PAIRED stages (the last stage of a forward real transform, the first of an inverse one)
+// give thread t the butterflies t and NB-t (thread 0: 0 and NB/2), so the thread holds
+// both z[k] and z[N-k] of every bin pair its outputs (inputs) touch and the real split
+// (merge) runs in registers, fused with the stage.
+template <class A, bool INV, int T, class RS, int S, bool PAIRED>
 struct FastStage {
     typedef typename A::store2 St;
     typedef Ops<A> O;
@@ -227,11 +264,21 @@ struct FastStage {
     static constexpr int m = RS::m(S);
     static constexpr int NB = N / p;
     static constexpr int P = N / (p * m);
-    static constexpr int J = (NB + T - 1) / T;
+    static constexpr int J = PAIRED ? 2 : (NB + T - 1) / T;
     static constexpr int PR = RS::PR(S);
     static constexpr int TWO = RS::tw_off(S);
+    static_assert(!PAIRED || (NB % 2 == 0 && T * 2 == NB), "paired stage needs N/p == 2*T");
 
-    __device__ __forceinline__ static bool live(int t, int j) { return (NB % T) == 0 || t + j * T < NB; }
+    __device__ __forceinline__ static bool live(int t, int j)
+    {
+        return PAIRED || (NB % T) == 0 || t + j * T < NB;
+    }
+    __device__ __forceinline__ static int bidx(int t, int j)
+    {
+        if (PAIRED)
+            return j == 0 ? t : (t == 0 ? NB / 2 : NB - t);
+        return t + j * T;
+    }
 
     template <bool GSRC>
     __device__ __forceinline__ static void gather(C (&v)[J][p], const St *__restrict__ src, long long src_el, int t,
@@ -241,7 +288,7 @@ struct FastStage {
         for (int j = 0; j < J; ++j) {
             if (!live(t, j))
                 break;
-            const int b = t + j * T;
+            const int b = bidx(t, j);
             const int u = b / P, o = b - (b / P) * P;
 #pragma unroll
             for (int r = 0; r < p; ++r) {
@@ -254,15 +301,64 @@ struct FastStage {
         }
     }
 
-    template <bool GDST>
-    __device__ __forceinline__ static void scatter(C (&v)[J][p], St *__restrict__ dst, long long dst_el,
-                                                   const St *__restrict__ stw, int t, bool valid)
+    // First stage of an inverse real transform: bins from HBM, Hermitian merge into the
+    // stage inputs z[r*NB + b] (m == 1, so P == NB).
+    __device__ __forceinline__ static void gather_merge(C (&v)[J][p], const St *__restrict__ bins, long long el,
+                                                        const St *__restrict__ rtw, int t, bool valid)
+    {
+        static_assert(m == 1, "the merge feeds the first stage");
+        auto X = [&](int k) { return valid ? O::load(bins[k * el]) : O::zero(); };
+        if (t == 0) {
+            // butterfly 0: z[r*NB], pairs (r, p-r); z[0] from DC and Nyquist
+            {
+                C x0 = X(0), xn = X(N);
+                v[0][0] = O::fixdiv(C{O::add(x0.r, xn.r), O::sub(x0.r, xn.r)}, 2);
+            }
+#pragma unroll
+            for (int r = 1; 2 * r <= p; ++r) {
+                const int k = r * NB;
+                C zk, znk;
+                merge_pair<A>(X(k), X(N - k), O::load(rtw[k - 1]), zk, znk);
+                if (2 * r != p)
+                    v[0][r] = zk;
+                v[0][p - r] = znk;
+            }
+            // butterfly NB/2: z[NB/2 + r*NB], pairs (r, p-1-r)
+#pragma unroll
+            for (int r = 0; 2 * r <= p - 1; ++r) {
+                const int k = NB / 2 + r * NB;
+                C zk, znk;
+                merge_pair<A>(X(k), X(N - k), O::load(rtw[k - 1]), zk, znk);
+                if (2 * r != p - 1)
+                    v[1][r] = zk;
+                v[1][p - 1 - r] = znk;
+            }
+            return;
+        }
+#pragma unroll
+        for (int r = 0; r < p; ++r) {
+            // z[k] is input r of butterfly t; z[N-k] is input p-1-r of butterfly NB-t
+            const int k = r * NB + t;
+            const int lo = k < N - k ? k : N - k;
+            C zlo, zhi;
+            merge_pair<A>(X(lo), X(N - lo), O::load(rtw[lo - 1]), zlo, zhi);
+            if (k == lo) {
+                v[0][r] = zlo;
+                v[1][p - 1 - r] = zhi;
+            } else {
+                v[0][r] = zhi;
+                v[1][p - 1 - r] = zlo;
+            }
+        }
+    }
+
+    __device__ __forceinline__ static void compute(C (&v)[J][p], const St *__restrict__ stw, int t)
     {
 #pragma unroll
         for (int j = 0; j < J; ++j) {
             if (!live(t, j))
                 break;
-            const int b = t + j * T;
+            const int b = bidx(t, j);
             const int u = b / P;
             const St *row = stw + TWO + u * PR;
             if constexpr (A::fixed) {
@@ -289,6 +385,18 @@ struct FastStage {
                 }
                 Dft<p, INV>::run(v[j]);
             }
+        }
+    }
+
+    template <bool GDST>
+    __device__ __forceinline__ static void store(C (&v)[J][p], St *__restrict__ dst, long long dst_el, int t,
+                                                 bool valid)
+    {
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            if (!live(t, j))
+                break;
+            const int b = bidx(t, j);
 #pragma unroll
             for (int q = 0; q < p; ++q) {
                 const int idx = b + q * NB;
@@ -301,39 +409,99 @@ struct FastStage {
             }
         }
     }
+
+    // Last stage of a forward real transform: outputs z[b + q*NB] of butterflies t and
+    // NB-t, split into bins k and N-k and written to HBM.
+    __device__ __forceinline__ static void store_split(C (&v)[J][p], St *__restrict__ out, long long el,
+                                                       const St *__restrict__ rtw, int t, bool valid)
+    {
+        if (!valid)
+            return;
+        auto put = [&](int k, C c) { out[k * el] = O::pack(c); };
+        if (t == 0) {
+            {
+                C dc = O::fixdiv(v[0][0], 2);
+                put(0, C{O::add(dc.r, dc.i), O::zero().r});
+                put(N, C{O::sub(dc.r, dc.i), O::zero().r});
+            }
+#pragma unroll
+            for (int q = 1; 2 * q <= p; ++q) {
+                const int k = q * NB;
+                C ok, onk;
+                split_pair<A>(v[0][q], v[0][p - q], O::load(rtw[k - 1]), ok, onk);
+                if (2 * q != p)
+                    put(k, ok);
+                put(N - k, onk);
+            }
+#pragma unroll
+            for (int q = 0; 2 * q <= p - 1; ++q) {
+                const int k = NB / 2 + q * NB;
+                C ok, onk;
+                split_pair<A>(v[1][q], v[1][p - 1 - q], O::load(rtw[k - 1]), ok, onk);
+                if (2 * q != p - 1)
+                    put(k, ok);
+                put(N - k, onk);
+            }
+            return;
+        }
+#pragma unroll
+        for (int q = 0; q < p; ++q) {
+            const int k = t + q * NB;  // output q of butterfly t; N-k is output p-1-q of NB-t
+            const int lo = k < N - k ? k : N - k;
+            C zlo = (k == lo) ? v[0][q] : v[1][p - 1 - q];
+            C zhi = (k == lo) ? v[1][p - 1 - q] : v[0][q];
+            C ok, onk;
+            split_pair<A>(zlo, zhi, O::load(rtw[lo - 1]), ok, onk);
+            put(lo, ok);
+            put(N - lo, onk);
+        }
+    }
 };
 
 // Runs stage S.  Double-buffered LDS (DB): stage S reads buffer S%2 and writes the other,
-// one barrier per stage.  Single buffer: gather, barrier, scatter, barrier.
-template <class A, bool INV, int T, int KIND, bool DB, class RS, int S>
+// one barrier per stage.  Single buffer: gather, barrier, scatter, barrier.  With FUSE the
+// real split / merge runs inside the last / first stage (paired butterflies).
+template <class A, bool INV, int T, int KIND, bool DB, bool FUSE, class RS, int S>
 __device__ __forceinline__ void fast_run_one(typename A::store2 *bufA, typename A::store2 *bufB,
                                              const typename A::store2 *gin, long long in_el,
                                              typename A::store2 *gout, long long out_el,
-                                             const typename A::store2 *stw, int t, bool valid)
+                                             const typename A::store2 *stw, const typename A::store2 *rtw, int t,
+                                             bool valid)
 {
     typedef typename A::store2 Sx;
-    typedef FastStage<A, INV, T, RS, S> F;
     constexpr int K = RS::K;
-    constexpr bool GSRC = (S == 0) && (KIND != FK_C2R);
-    constexpr bool GDST = (S == K - 1) && (KIND != FK_R2C);
+    constexpr bool SPLIT = FUSE && KIND == FK_R2C && S == K - 1;
+    constexpr bool MERGE = FUSE && KIND == FK_C2R && S == 0;
+    typedef FastStage<A, INV, T, RS, S, SPLIT || MERGE> F;
+    constexpr bool GSRC = (S == 0) && (KIND != FK_C2R || MERGE);
+    constexpr bool GDST = (S == K - 1) && (KIND != FK_R2C || SPLIT);
     const Sx *src = GSRC ? gin : ((!DB || S % 2 == 0) ? bufA : bufB);
     Sx *dst = GDST ? gout : ((!DB || S % 2 == 1) ? bufA : bufB);
     typename F::C v[F::J][F::p];
-    F::template gather<GSRC>(v, src, GSRC ? in_el : 1, t, valid);
+    if constexpr (MERGE)
+        F::gather_merge(v, gin, in_el, rtw, t, valid);
+    else
+        F::template gather<GSRC>(v, src, GSRC ? in_el : 1, t, valid);
     if constexpr (!DB && !GSRC && !GDST)
         __syncthreads();  // every input read before the buffer is overwritten
-    F::template scatter<GDST>(v, dst, GDST ? out_el : 1, stw, t, valid);
+    F::compute(v, stw, t);
+    if constexpr (SPLIT)
+        F::store_split(v, gout, out_el, rtw, t, valid);
+    else
+        F::template store<GDST>(v, dst, GDST ? out_el : 1, t, valid);
     if constexpr (!GDST)
         __syncthreads();
 }
 
-template <class A, bool INV, int T, int KIND, bool DB, class RS, int... S>
+template <class A, bool INV, int T, int KIND, bool DB, bool FUSE, class RS, int... S>
 __device__ __forceinline__ void fast_run_all(std::integer_sequence<int, S...>, typename A::store2 *bufA,
                                              typename A::store2 *bufB, const typename A::store2 *gin,
                                              long long in_el, typename A::store2 *gout, long long out_el,
-                                             const typename A::store2 *stw, int t, bool valid)
+                                             const typename A::store2 *stw, const typename A::store2 *rtw, int t,
+                                             bool valid)
 {
-    (fast_run_one<A, INV, T, KIND, DB, RS, S>(bufA, bufB, gin, in_el, gout, out_el, stw, t, valid), ...);
+    (fast_run_one<A, INV, T, KIND, DB, FUSE, RS, S>(bufA, bufB, gin, in_el, gout, out_el, stw, rtw, t, valid),
+     ...);
 }
 
 // buffer holding the result of the last stage (input of the real split)
@@ -343,7 +511,7 @@ struct LastBuf {
 };
 
 // Dynamic LDS: (DB ? 2 : 1) * LPB * N complex values.
-template <class A, int KIND, bool INV, int T, int LPB, bool DB, class RS>
+template <class A, int KIND, bool INV, int T, int LPB, bool DB, bool FUSE, class RS>
 __global__ void __launch_bounds__(T *LPB) k_fast(FastArgs a)
 {
     typedef typename A::store2 S;
@@ -363,7 +531,7 @@ __global__ void __launch_bounds__(T *LPB) k_fast(FastArgs a)
     const S *stw = reinterpret_cast<const S *>(__builtin_assume_aligned(a.stw, 16));
     const S *rtw = reinterpret_cast<const S *>(a.rtw);
 
-    if constexpr (KIND == FK_C2R) {
+    if constexpr (KIND == FK_C2R && !FUSE) {
         // Hermitian merge into bufA (kiss_fftr.c:131-153); bins read from HBM
         constexpr int NK = N / 2 + 1;
         constexpr int JK = (NK + T - 1) / T;
@@ -395,10 +563,10 @@ __global__ void __launch_bounds__(T *LPB) k_fast(FastArgs a)
         __syncthreads();
     }
 
-    fast_run_all<A, INV, T, KIND, DB, RS>(std::make_integer_sequence<int, K>{}, bufA, bufB, gin, a.in_el, gout,
-                                          a.out_el, stw, t, valid);
+    fast_run_all<A, INV, T, KIND, DB, FUSE, RS>(std::make_integer_sequence<int, K>{}, bufA, bufB, gin, a.in_el,
+                                                gout, a.out_el, stw, rtw, t, valid);
 
-    if constexpr (KIND == FK_R2C) {
+    if constexpr (KIND == FK_R2C && !FUSE) {
         // split (kiss_fftr.c:88-116) from the LDS buffer the last stage wrote
         const S *z = LastBuf<DB, K>::is_a ? bufA : bufB;
         constexpr int NK = N / 2 + 1;

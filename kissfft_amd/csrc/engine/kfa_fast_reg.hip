@@ -15,20 +15,21 @@
 
 namespace kfa {
 
-template <class A, int KIND, bool INV, int T, int LPB, bool DB, class RS>
+template <class A, int KIND, bool INV, int T, int LPB, bool DB, bool FUSE, class RS>
 static int launch_fast(const FastArgs &a, hipStream_t s)
 {
     typedef typename A::store2 S;
     const size_t lds = (DB ? 2 : 1) * (size_t)LPB * RS::N() * sizeof(S);
     static std::once_flag once;
     std::call_once(once, [=] {
-        (void)hipFuncSetAttribute((const void *)k_fast<A, KIND, INV, T, LPB, DB, RS>,
+        (void)hipFuncSetAttribute((const void *)k_fast<A, KIND, INV, T, LPB, DB, FUSE, RS>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     });
     const long long blocks = (a.nlines + LPB - 1) / LPB;
     if (blocks <= 0)
         return 0;
-    hipLaunchKernelGGL((k_fast<A, KIND, INV, T, LPB, DB, RS>), dim3((unsigned)blocks), dim3(T * LPB), lds, s, a);
+    hipLaunchKernelGGL((k_fast<A, KIND, INV, T, LPB, DB, FUSE, RS>), dim3((unsigned)blocks), dim3(T * LPB), lds, s,
+                       a);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
@@ -38,7 +39,7 @@ struct TwPad {
     static constexpr int G = 16 / (int)sizeof(typename A::store2);
 };
 
-template <class A, int KIND, int T, int LPB, bool DB, int... R>
+template <class A, int KIND, int T, int LPB, bool DB, bool FUSE, int... R>
 static void add_variant(std::vector<FastVariant> &v, int type, const char *name)
 {
     constexpr int G = TwPad<A>::G;
@@ -61,17 +62,26 @@ static void add_variant(std::vector<FastVariant> &v, int type, const char *name)
         f.tw_pad = G;
         f.tw_len = RS::tw_len();
         f.name = name;
-        f.launch = inv ? &launch_fast<A, KIND, true, T, LPB, DB, RS> : &launch_fast<A, KIND, false, T, LPB, DB, RS>;
+        f.launch = inv ? &launch_fast<A, KIND, true, T, LPB, DB, FUSE, RS>
+                       : &launch_fast<A, KIND, false, T, LPB, DB, FUSE, RS>;
         v.push_back(f);
     }
 }
 
-// forward and inverse real variants of one schedule
+// forward and inverse real variants of one schedule, split / merge in a separate LDS pass
 template <class A, int T, int LPB, bool DB, int... R>
 static void add_real(std::vector<FastVariant> &v, int type, const char *name)
 {
-    add_variant<A, FK_R2C, T, LPB, DB, R...>(v, type, name);
-    add_variant<A, FK_C2R, T, LPB, DB, R...>(v, type, name);
+    add_variant<A, FK_R2C, T, LPB, DB, false, R...>(v, type, name);
+    add_variant<A, FK_C2R, T, LPB, DB, false, R...>(v, type, name);
+}
+
+// a real variant with the split (forward) / merge (inverse) fused into the last / first
+// stage; N / (that stage's radix) must equal 2*T
+template <class A, int KIND, int T, int LPB, int... R>
+static void add_fused(std::vector<FastVariant> &v, int type, const char *name)
+{
+    add_variant<A, KIND, T, LPB, false, true, R...>(v, type, name);
 }
 
 const std::vector<FastVariant> &fast_registry()
@@ -92,31 +102,49 @@ const std::vector<FastVariant> &fast_registry()
         add_real<F32, 250, 1, false, 5, 5, 5, 4, 4>(*v, KFA_F32, "real2000_55544_t250_sb");
         add_real<F32, 125, 2, false, 16, 5, 5, 5>(*v, KFA_F32, "real2000_16555_t125x2_sb");
         add_real<F32, 250, 1, false, 8, 2, 5, 5, 5>(*v, KFA_F32, "real2000_82555_t250_sb");
+        add_fused<F32, FK_R2C, 128, 1, 16, 16, 8>(*v, KFA_F32, "r2c2048_16168_t128_fused");
+        add_fused<F32, FK_C2R, 128, 1, 8, 16, 16>(*v, KFA_F32, "c2r2048_81616_t128_fused");
+        add_fused<F32, FK_R2C, 128, 2, 16, 16, 8>(*v, KFA_F32, "r2c2048_16168_t128x2_fused");
+        add_fused<F32, FK_C2R, 128, 2, 8, 16, 16>(*v, KFA_F32, "c2r2048_81616_t128x2_fused");
+        add_fused<F32, FK_R2C, 256, 1, 8, 8, 8, 4>(*v, KFA_F32, "r2c2048_8884_t256_fused");
+        add_fused<F32, FK_C2R, 256, 1, 4, 8, 8, 8>(*v, KFA_F32, "c2r2048_4888_t256_fused");
+        add_fused<F32, FK_R2C, 250, 1, 5, 5, 5, 4, 4>(*v, KFA_F32, "r2c2000_55544_t250_fused");
+        add_fused<F32, FK_C2R, 250, 1, 4, 4, 5, 5, 5>(*v, KFA_F32, "c2r2000_44555_t250_fused");
+        add_fused<F32, FK_R2C, 200, 1, 16, 5, 5, 5>(*v, KFA_F32, "r2c2000_16555_t200_fused");
+        add_fused<F32, FK_C2R, 200, 1, 5, 5, 5, 16>(*v, KFA_F32, "c2r2000_55516_t200_fused");
+        add_fused<F32, FK_R2C, 200, 1, 8, 2, 5, 5, 5>(*v, KFA_F32, "r2c2000_82555_t200_fused");
+        add_fused<F32, FK_C2R, 200, 1, 5, 5, 5, 2, 8>(*v, KFA_F32, "c2r2000_55528_t200_fused");
         // N = 1000 (half 500), the real axis of the 2-D workload
         add_real<F32, 50, 4, false, 4, 5, 5, 5>(*v, KFA_F32, "real500_4555_t50x4_sb");
         add_real<F32, 125, 2, false, 4, 5, 5, 5>(*v, KFA_F32, "real500_4555_t125x2_sb");
+        add_fused<F32, FK_R2C, 50, 4, 4, 5, 5, 5>(*v, KFA_F32, "r2c500_4555_t50x4_fused");
+        add_fused<F32, FK_C2R, 50, 4, 5, 5, 5, 4>(*v, KFA_F32, "c2r500_5554_t50x4_fused");
         // ---- float complex ----
-        add_variant<F32, FK_C2C, 240, 1, false, 4, 4, 3, 5, 5>(*v, KFA_F32, "c2c1200_44355_t240_sb");
-        add_variant<F32, FK_C2C, 240, 1, true, 4, 4, 3, 5, 5>(*v, KFA_F32, "c2c1200_44355_t240_db");
-        add_variant<F32, FK_C2C, 120, 2, false, 5, 5, 3, 16>(*v, KFA_F32, "c2c1200_55316_t120x2_sb");
-        add_variant<F32, FK_C2C, 128, 1, false, 16, 16, 8>(*v, KFA_F32, "c2c2048_16168_t128_sb");
-        add_variant<F32, FK_C2C, 256, 1, false, 8, 8, 8, 4>(*v, KFA_F32, "c2c2048_8884_t256_sb");
-        add_variant<F32, FK_C2C, 64, 2, false, 16, 16, 4>(*v, KFA_F32, "c2c1024_16164_t64x2_sb");
-        add_variant<F32, FK_C2C, 128, 2, false, 8, 8, 4, 4>(*v, KFA_F32, "c2c1024_8844_t128x2_sb");
-        add_variant<F32, FK_C2C, 64, 4, false, 8, 8, 8>(*v, KFA_F32, "c2c512_888_t64x4_sb");
-        add_variant<F32, FK_C2C, 16, 16, false, 16, 16>(*v, KFA_F32, "c2c256_1616_t16x16_sb");
-        add_variant<F32, FK_C2C, 64, 4, false, 4, 4, 4, 4>(*v, KFA_F32, "c2c256_4444_t64x4_sb");
-        add_variant<F32, FK_C2C, 100, 2, false, 5, 5, 5, 4, 2>(*v, KFA_F32, "c2c1000_55542_t100x2_sb");
-        add_variant<F32, FK_C2C, 125, 2, false, 5, 5, 5, 8>(*v, KFA_F32, "c2c1000_5558_t125x2_sb");
-        add_variant<F32, FK_C2C, 192, 1, false, 3, 16, 8, 8>(*v, KFA_F32, "c2c3072_31688_t192_sb");
-        add_variant<F32, FK_C2C, 256, 1, false, 16, 16, 16>(*v, KFA_F32, "c2c4096_161616_t256_sb");
+        add_variant<F32, FK_C2C, 240, 1, false, false, 4, 4, 3, 5, 5>(*v, KFA_F32, "c2c1200_44355_t240_sb");
+        add_variant<F32, FK_C2C, 240, 1, true, false, 4, 4, 3, 5, 5>(*v, KFA_F32, "c2c1200_44355_t240_db");
+        add_variant<F32, FK_C2C, 120, 2, false, false, 5, 5, 3, 16>(*v, KFA_F32, "c2c1200_55316_t120x2_sb");
+        add_variant<F32, FK_C2C, 128, 1, false, false, 16, 16, 8>(*v, KFA_F32, "c2c2048_16168_t128_sb");
+        add_variant<F32, FK_C2C, 256, 1, false, false, 8, 8, 8, 4>(*v, KFA_F32, "c2c2048_8884_t256_sb");
+        add_variant<F32, FK_C2C, 64, 2, false, false, 16, 16, 4>(*v, KFA_F32, "c2c1024_16164_t64x2_sb");
+        add_variant<F32, FK_C2C, 128, 2, false, false, 8, 8, 4, 4>(*v, KFA_F32, "c2c1024_8844_t128x2_sb");
+        add_variant<F32, FK_C2C, 64, 4, false, false, 8, 8, 8>(*v, KFA_F32, "c2c512_888_t64x4_sb");
+        add_variant<F32, FK_C2C, 16, 16, false, false, 16, 16>(*v, KFA_F32, "c2c256_1616_t16x16_sb");
+        add_variant<F32, FK_C2C, 64, 4, false, false, 4, 4, 4, 4>(*v, KFA_F32, "c2c256_4444_t64x4_sb");
+        add_variant<F32, FK_C2C, 100, 2, false, false, 5, 5, 5, 4, 2>(*v, KFA_F32, "c2c1000_55542_t100x2_sb");
+        add_variant<F32, FK_C2C, 125, 2, false, false, 5, 5, 5, 8>(*v, KFA_F32, "c2c1000_5558_t125x2_sb");
+        add_variant<F32, FK_C2C, 192, 1, false, false, 3, 16, 8, 8>(*v, KFA_F32, "c2c3072_31688_t192_sb");
+        add_variant<F32, FK_C2C, 256, 1, false, false, 16, 16, 16>(*v, KFA_F32, "c2c4096_161616_t256_sb");
         // ---- Q15 (reference schedules) ----
-        add_variant<Q15, FK_C2C, 128, 2, true, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15c2c512_t128x2_db");
-        add_variant<Q15, FK_C2C, 128, 2, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15c2c512_t128x2_sb");
-        add_variant<Q15, FK_C2C, 64, 4, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15c2c512_t64x4_sb");
-        add_variant<Q15, FK_C2C, 128, 4, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15c2c512_t128x4_sb");
+        add_variant<Q15, FK_C2C, 128, 2, true, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15c2c512_t128x2_db");
+        add_variant<Q15, FK_C2C, 128, 2, false, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15c2c512_t128x2_sb");
+        add_variant<Q15, FK_C2C, 64, 4, false, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15c2c512_t64x4_sb");
+        add_variant<Q15, FK_C2C, 128, 4, false, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15c2c512_t128x4_sb");
         add_real<Q15, 128, 2, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15real512_t128x2_sb");
         add_real<Q15, 256, 1, false, 2, 4, 4, 4, 4, 4>(*v, KFA_Q15, "q15real2048_t256_sb");
+        add_fused<Q15, FK_R2C, 64, 4, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15r2c512_t64x4_fused");
+        add_fused<Q15, FK_C2R, 128, 2, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15c2r512_t128x2_fused");
+        add_fused<Q15, FK_R2C, 256, 1, 2, 4, 4, 4, 4, 4>(*v, KFA_Q15, "q15r2c2048_t256_fused");
+        add_fused<Q15, FK_C2R, 512, 1, 2, 4, 4, 4, 4, 4>(*v, KFA_Q15, "q15c2r2048_t512_fused");
         return v;
     }();
     return *reg;

@@ -26,14 +26,19 @@ REG = os.path.join(ROOT, "kissfft_amd", "csrc", "engine", "kfa_fast_reg.hip")
 def registry():
     src = open(REG).read()
     out = []
-    for m in re.finditer(r"add_(real|variant)<(\w+),\s*(?:(FK_\w+),\s*)?(\d+),\s*(\d+),\s*(true|false),\s*([\d,\s]+)>"
-                         r"\(\*v,\s*KFA_\w+,\s*\"(\w+)\"\)", src):
-        kind_fn, typ, fk, T, LPB, db, rad, name = m.groups()
-        radices = [int(r) for r in rad.replace(" ", "").split(",") if r]
-        n = int(np.prod(radices))
-        kinds = ["r2c", "c2r"] if kind_fn == "real" else [{"FK_C2C": "c2c", "FK_R2C": "r2c", "FK_C2R": "c2r"}[fk]]
+    fk = {"FK_C2C": "c2c", "FK_R2C": "r2c", "FK_C2R": "c2r"}
+    for m in re.finditer(r"add_(real|variant|fused)<([^>]*)>\(\*v,\s*KFA_\w+,\s*\"(\w+)\"\)", src):
+        fn, targs, name = m.groups()
+        a = [x.strip() for x in targs.split(",")]
+        if fn == "real":  # <A, T, LPB, DB, radices...>
+            kinds, radices = ["r2c", "c2r"], a[4:]
+        elif fn == "variant":  # <A, FK, T, LPB, DB, FUSE, radices...>
+            kinds, radices = [fk[a[1]]], a[6:]
+        else:  # fused: <A, FK, T, LPB, radices...>
+            kinds, radices = [fk[a[1]]], a[4:]
+        n = int(np.prod([int(r) for r in radices]))
         for k in kinds:
-            out.append((typ, k, n, name))
+            out.append((a[0], k, n, name))
     return out
 
 
