@@ -68,12 +68,85 @@ def cpu_baseline(threads: int) -> dict | None:
     }
 
 
+def _time(fn, stream, reps: int) -> float:
+    import torch
+    fn()
+    stream.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / 1e3 / reps
+
+
+def other_configs(lib, stream, dev) -> dict:
+    """The reference's other workloads, each timed on its own (inputs resident in HBM)."""
+    import torch
+
+    import kissfft_amd
+
+    out = {}
+
+    def rec(name, secs, nbytes, transforms, **extra):
+        out[name] = {"us": round(secs * 1e6, 1), "GBps": round(nbytes / secs / 1e9, 1),
+                     "pct_hbm_peak": round(100 * nbytes / secs / 1e9 / HBM_PEAK_GBPS, 2),
+                     "transforms_per_s": round(transforms / secs, 1), **extra}
+
+    with torch.cuda.stream(stream):
+        # complex float N=1200, 256 transforms (latency-bound: one launch of 4.9 MB)
+        x = torch.rand(256, 1200, 2, device=dev) * 2 - 1
+        y = torch.empty_like(x)
+    cfg = lib.fft_alloc(1200)
+    t = _time(lambda: lib.fft_batch(cfg, x, y, 256, 1200, 1200, stream=stream), stream, 50)
+    rec("c2c_float_1200_x256", t, 256 * 16 * 1200, 256)
+
+    # Q15 complex N=512, 2^20 transforms
+    lq = kissfft_amd.load("int16_t")
+    B = 1 << 20
+    with torch.cuda.stream(stream):
+        xq = torch.randint(-16384, 16384, (B, 512, 2), dtype=torch.int16, device=dev)
+        yq = torch.empty_like(xq)
+    cq = lq.fft_alloc(512)
+    t = _time(lambda: lq.fft_batch(cq, xq, yq, B, 512, 512, stream=stream), stream, 5)
+    rec("c2c_q15_512_x1048576", t, B * 8 * 512, B)
+    del xq, yq
+
+    # 3-D complex float 64x96x120, 16 transforms
+    dims = (64, 96, 120)
+    with torch.cuda.stream(stream):
+        x3 = torch.rand((16,) + dims + (2,), device=dev) * 2 - 1
+        y3 = torch.empty_like(x3)
+    c3 = lib.fftnd_alloc(dims)
+    t = _time(lambda: lib.fftnd_batch(c3, x3, y3, 16, stream=stream), stream, 10)
+    rec("nd_float_64x96x120_x16", t, 16 * 16 * 64 * 96 * 120, 16)
+    del x3, y3
+
+    # 2-D real float 1024x1000, 32 transforms (forward and inverse)
+    d2 = (1024, 1000)
+    with torch.cuda.stream(stream):
+        xr = torch.randn((32,) + d2, device=dev)
+        Xr = torch.empty(32, 1024, 501, 2, device=dev)
+        yr = torch.empty_like(xr)
+    cf, ci = lib.fftndr_alloc(d2), lib.fftndr_alloc(d2, True)
+    nb = 32 * (4 * 1024 * 1000 + 8 * 1024 * 501)
+    t = _time(lambda: lib.fftndr_batch(cf, xr, Xr, 32, stream=stream), stream, 10)
+    rec("ndr_float_1024x1000_x32_fwd", t, nb, 32)
+    t = _time(lambda: lib.fftndri_batch(ci, Xr, yr, 32, stream=stream), stream, 10)
+    rec("ndr_float_1024x1000_x32_inv", t, nb, 32)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-other-configs", action="store_true",
+                    help="skip the secondary workloads reported under other_configs")
     ap.add_argument("--pmc-run", action="store_true",
                     help="for rocprofv3 --pmc: one plain copy of a known size (the counter scale), "
                          "then one forward and one inverse N=4096 launch, and exit")
@@ -166,6 +239,8 @@ def main() -> int:
             s = ev0.elapsed_time(ev1) / 1e3 / 5
             per_kernel[f"{name}_{n}"] = {"us": round(s * 1e6, 1), "GBps": round(alg_bytes(n) * BATCH / s / 1e9, 1)}
 
+    others = {} if args.no_other_configs else other_configs(lib, stream, dev)
+
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic_r2c_4096.json")
     if os.path.exists(tfile):
@@ -200,6 +275,7 @@ def main() -> int:
             "nfft": list(SIZES),
         },
         "pct_hbm_peak": round(100 * value / HBM_PEAK_GBPS, 2),
+        "other_configs": others,
         "transforms_per_s": round(transforms_per_s, 1),
         "per_kernel": per_kernel,
         "roofline": {

@@ -21,7 +21,7 @@ import numpy as np
 
 from . import build as _build
 
-__all__ = ["Library", "load", "SCALARS", "KissFFTError"]
+__all__ = ["Library", "Server", "load", "SCALARS", "KissFFTError"]
 
 SCALARS = {"float": np.float32, "int16_t": np.int16, "int32_t": np.int32}
 
@@ -66,7 +66,8 @@ class Cfg:
         return ctypes.c_void_p(self.ptr)
 
     def __del__(self):
-        if getattr(self, "ptr", 0):
+        # at interpreter exit module globals may already be gone
+        if getattr(self, "ptr", 0) and _libc is not None and getattr(_libc, "free", None) is not None:
             _libc.free(self.ptr)
             self.ptr = 0
 
@@ -137,6 +138,13 @@ class Library:
             "kiss_fft_last_error": ([], ctypes.c_char_p),
             "kiss_fft_device_available": ([], ci),
             "kiss_fft_get_engine_stats": ([vp], None),
+            "kiss_fft_server_create": ([ll, ci, ci], vp),
+            "kiss_fft_server_submit": ([vp, ci, ci, vp, vp, ll], ll),
+            "kiss_fft_server_wait": ([vp, ll, ctypes.POINTER(ctypes.c_double)], ci),
+            "kiss_fft_server_drain": ([vp], ci),
+            "kiss_fft_server_get_stats": ([vp, ctypes.POINTER(ServerStats)], ci),
+            "kiss_fft_server_reset_stats": ([vp], None),
+            "kiss_fft_server_destroy": ([vp], None),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -260,6 +268,71 @@ class Library:
     def stream_sync(self, stream=None) -> None:
         self.check(self.L.kiss_fft_stream_sync(_stream_ptr(stream) if stream is not None else None),
                    "kiss_fft_stream_sync")
+
+    def server(self, hbm_budget_bytes: int = 0, streams: int = 0, max_group: int = 0) -> "Server":
+        return Server(self, hbm_budget_bytes, streams, max_group)
+
+
+class ServerStats(ctypes.Structure):
+    """kiss_fft_server_stats (include/kiss_fft_serve.h)."""
+    _fields_ = [(n, ctypes.c_longlong) for n in ("submitted", "completed", "failed", "launches",
+                                                 "grouped_requests", "bytes", "budget_bytes",
+                                                 "in_use_bytes")] + \
+               [(n, ctypes.c_double) for n in ("p50_s", "p99_s", "max_s", "mean_s", "span_s")]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class Server:
+    """A kiss_fft_server: requests of mixed kind and length, batched on the device.
+
+    Kinds are C2C_FWD, C2C_INV, R2C and C2R.  Buffers are numpy arrays (staged through the
+    server's HBM budget) or device tensors (used in place); they must stay alive until the
+    request's wait() returns."""
+    C2C_FWD, C2C_INV, R2C, C2R = 0, 1, 2, 3
+
+    def __init__(self, lib: Library, hbm_budget_bytes: int = 0, streams: int = 0, max_group: int = 0):
+        self.lib = lib
+        self.ptr = lib.L.kiss_fft_server_create(int(hbm_budget_bytes), int(streams), int(max_group))
+        if not self.ptr:
+            raise KissFFTError(f"kiss_fft_server_create failed: {lib.last_error()}")
+
+    def submit(self, kind: int, nfft: int, fin, fout, howmany: int) -> int:
+        t = self.lib.L.kiss_fft_server_submit(self.ptr, kind, nfft, _addr(fin), _addr(fout), howmany)
+        if t <= 0:
+            raise KissFFTError(f"kiss_fft_server_submit failed ({t}): {self.lib.last_error()}")
+        return t
+
+    def wait(self, ticket: int) -> float:
+        """Latency in seconds (submit to completion); raises if the request failed."""
+        lat = ctypes.c_double()
+        rc = self.lib.L.kiss_fft_server_wait(self.ptr, ticket, ctypes.byref(lat))
+        if rc != 0:
+            raise KissFFTError(f"request {ticket} failed ({rc}): {self.lib.last_error()}")
+        return lat.value
+
+    def drain(self) -> None:
+        self.lib.check(self.lib.L.kiss_fft_server_drain(self.ptr), "kiss_fft_server_drain")
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        self.lib.L.kiss_fft_server_get_stats(self.ptr, ctypes.byref(st))
+        return st.as_dict()
+
+    def reset_stats(self) -> None:
+        self.lib.L.kiss_fft_server_reset_stats(self.ptr)
+
+    def close(self) -> None:
+        if getattr(self, "ptr", None):
+            self.lib.L.kiss_fft_server_destroy(self.ptr)
+            self.ptr = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def _stream_ptr(stream):

@@ -21,6 +21,7 @@
 
 #include "kfa_arith.hpp"
 #include "kfa_lines.hpp"
+#include "kfa_q15p.hpp"
 
 namespace kfa {
 
@@ -55,6 +56,17 @@ struct Radices {
     static constexpr int tw_len() { return tw_off(K); }
 };
 
+// One launch may cover several separate buffers ("segments", e.g. the requests of a
+// serving batch): segment s owns lines [first[s], first[s+1]) and reads / writes them at
+// in[s] / out[s] with the common distances.  nseg == 0: one buffer, in / out below.
+struct FastSeg {
+    long long first;  // first line of the segment
+    const void *in;
+    void *out;
+};
+
+#define KFA_MAX_SEG 64
+
 struct FastArgs {
     const void *in;
     void *out;
@@ -63,7 +75,31 @@ struct FastArgs {
     long long nlines;
     long long in_hi, in_el;
     long long out_hi, out_el;
+    int nseg;                    // 0: one buffer (in / out); else segments below
+    FastSeg seg[KFA_MAX_SEG];    // ascending `first`, seg[0].first == 0 (passed by value)
 };
+
+// Base pointers of global line gl: its segment's buffers and its index inside them.
+template <class S>
+__device__ __forceinline__ void line_base(const FastArgs &a, long long gl, const S *&in, S *&out)
+{
+    if (a.nseg <= 0) {
+        in = reinterpret_cast<const S *>(a.in) + gl * a.in_hi;
+        out = reinterpret_cast<S *>(a.out) + gl * a.out_hi;
+        return;
+    }
+    int lo = 0, hi = a.nseg - 1;  // last segment with first <= gl
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.seg[mid].first <= gl)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const long long k = gl - a.seg[lo].first;
+    in = reinterpret_cast<const S *>(a.seg[lo].in) + k * a.in_hi;
+    out = reinterpret_cast<S *>(a.seg[lo].out) + k * a.out_hi;
+}
 
 // ------------------------------------------------------------------------------------
 // Float radix-p DFTs on registers.  Forward uses exp(-2*pi*i/p); INV the conjugate.
@@ -361,7 +397,35 @@ struct FastStage {
             const int b = bidx(t, j);
             const int u = b / P;
             const St *row = stw + TWO + u * PR;
-            if constexpr (A::fixed) {
+            if constexpr (IsPacked<A>::value) {
+                // stage table of pre-arranged twiddle word pairs (see kfa_q15p.hpp)
+                const uint2 *tab = reinterpret_cast<const uint2 *>(stw);
+                const uint2 *trow = tab + TWO + u * PR;
+                uint2 w[p > 1 ? p - 1 : 1];
+#pragma unroll
+                for (int r = 1; r < p; ++r)
+                    w[r - 1] = trow[r - 1];
+                unsigned x[p];
+#pragma unroll
+                for (int r = 0; r < p; ++r)
+                    x[r] = v[j][r].w;
+                if constexpr (p == 2) {
+                    q15p::bfly2(x, w);
+                } else if constexpr (p == 4) {
+                    q15p::bfly4<INV>(x, w);
+                } else if constexpr (p == 3) {
+                    const uint2 ya = tab[TWO + m * PR];
+                    q15p::bfly3(x, w, q15p::scal(q15p::lo16(ya.y)));
+                } else {
+                    const uint2 ya = tab[TWO + m * PR], yb = tab[TWO + m * PR + 1];
+                    q15p::Y5 y{q15p::scal(q15p::lo16(ya.x)), q15p::scal(q15p::lo16(ya.y)),
+                               q15p::scal(q15p::lo16(yb.x)), q15p::scal(q15p::lo16(yb.y))};
+                    q15p::bfly5(x, w, y);
+                }
+#pragma unroll
+                for (int r = 0; r < p; ++r)
+                    v[j][r].w = x[r];
+            } else if constexpr (A::fixed) {
                 C w[p > 1 ? p - 1 : 1];
 #pragma unroll
                 for (int r = 1; r < p; ++r)
@@ -526,8 +590,9 @@ __global__ void __launch_bounds__(T *LPB) k_fast(FastArgs a)
     const bool valid = gl < a.nlines;
     S *bufA = reinterpret_cast<S *>(kfa_fast_smem) + line * N;
     S *bufB = DB ? reinterpret_cast<S *>(kfa_fast_smem) + (LPB + line) * N : bufA;
-    const S *gin = reinterpret_cast<const S *>(a.in) + (valid ? gl * a.in_hi : 0);
-    S *gout = reinterpret_cast<S *>(a.out) + (valid ? gl * a.out_hi : 0);
+    const S *gin;
+    S *gout;
+    line_base<S>(a, valid ? gl : 0, gin, gout);
     const S *stw = reinterpret_cast<const S *>(__builtin_assume_aligned(a.stw, 16));
     const S *rtw = reinterpret_cast<const S *>(a.rtw);
 

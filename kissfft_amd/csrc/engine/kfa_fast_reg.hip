@@ -37,6 +37,12 @@ static int launch_fast(const FastArgs &a, hipStream_t s)
 template <class A>
 struct TwPad {
     static constexpr int G = 16 / (int)sizeof(typename A::store2);
+    static constexpr int fmt = 0;
+};
+template <>
+struct TwPad<Q15P> {
+    static constexpr int G = 2;  // 8-byte word-pair entries
+    static constexpr int fmt = 1;
 };
 
 template <class A, int KIND, int T, int LPB, bool DB, bool FUSE, int... R>
@@ -60,6 +66,7 @@ static void add_variant(std::vector<FastVariant> &v, int type, const char *name)
         f.LPB = LPB;
         f.radices = {R...};
         f.tw_pad = G;
+        f.tw_fmt = TwPad<A>::fmt;
         f.tw_len = RS::tw_len();
         f.name = name;
         f.launch = inv ? &launch_fast<A, KIND, true, T, LPB, DB, FUSE, RS>
@@ -134,6 +141,14 @@ const std::vector<FastVariant> &fast_registry()
         add_variant<F32, FK_C2C, 125, 2, false, false, 5, 5, 5, 8>(*v, KFA_F32, "c2c1000_5558_t125x2_sb");
         add_variant<F32, FK_C2C, 192, 1, false, false, 3, 16, 8, 8>(*v, KFA_F32, "c2c3072_31688_t192_sb");
         add_variant<F32, FK_C2C, 256, 1, false, false, 16, 16, 16>(*v, KFA_F32, "c2c4096_161616_t256_sb");
+        // ---- Q15 (reference schedules), packed-word butterflies ----
+        add_variant<Q15P, FK_C2C, 64, 4, false, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c512_t64x4_sb");
+        add_variant<Q15P, FK_C2C, 128, 2, false, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c512_t128x2_sb");
+        add_variant<Q15P, FK_C2C, 32, 8, false, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c512_t32x8_sb");
+        add_variant<Q15P, FK_C2C, 64, 4, true, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c512_t64x4_db");
+        add_variant<Q15P, FK_C2C, 240, 1, false, false, 5, 5, 3, 4, 4>(*v, KFA_Q15, "q15p_c2c1200_t240_sb");
+        add_variant<Q15P, FK_C2C, 64, 4, false, false, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c256_t64x4_sb");
+        add_variant<Q15P, FK_C2C, 128, 2, false, false, 4, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c1024_t128x2_sb");
         // ---- Q15 (reference schedules) ----
         add_variant<Q15, FK_C2C, 128, 2, true, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15c2c512_t128x2_db");
         add_variant<Q15, FK_C2C, 128, 2, false, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15c2c512_t128x2_sb");

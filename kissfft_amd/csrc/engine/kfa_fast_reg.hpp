@@ -20,6 +20,7 @@ struct FastVariant {
     int LPB;      // lines per workgroup
     std::vector<int> radices;  // execution order, innermost first
     int tw_pad;   // stage-table rows padded to a multiple of this many entries
+    int tw_fmt;   // 0: entries as stored complex values; 1: Q15 word pairs (w.r,-w.i),(w.i,w.r)
     int tw_len;   // entries of the per-stage twiddle table
     const char *name;
     int (*launch)(const FastArgs &, hipStream_t);

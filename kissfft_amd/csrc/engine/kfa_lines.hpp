@@ -136,6 +136,52 @@ __device__ __forceinline__ void run_stage(const LinePlan &pl, const StageDesc &s
 }
 
 // Real-input split after the half-length transform, kiss_fftr.c:88-116.
+// DC/Nyquist pair from Z[0]:
+template <class A>
+__device__ __forceinline__ void split_dc(Cx<A> z0, Cx<A> &x0, Cx<A> &xn)
+{
+    typedef Ops<A> O;
+    const Cx<A> dc = O::fixdiv(z0, 2);
+    x0 = Cx<A>{O::add(dc.r, dc.i), O::zero().r};
+    xn = Cx<A>{O::sub(dc.r, dc.i), O::zero().r};
+}
+
+// X[k] and X[nc-k] (0 < k <= nc/2) from Z[k], Z[nc-k] and super twiddle tw = stw[k-1].
+template <class A>
+__device__ __forceinline__ void split_k(Cx<A> zk, Cx<A> znk, Cx<A> tw, Cx<A> &xk, Cx<A> &xnk)
+{
+    typedef Ops<A> O;
+    Cx<A> a = O::fixdiv(zk, 2);
+    Cx<A> b = O::fixdiv(Cx<A>{znk.r, O::neg(znk.i)}, 2);
+    const Cx<A> f1 = O::cadd(a, b), f2 = O::csub(a, b);
+    const Cx<A> t = O::cmul(f2, tw);
+    xk = Cx<A>{O::half_add(f1.r, t.r), O::half_add(f1.i, t.i)};
+    xnk = Cx<A>{O::half_sub(f1.r, t.r), O::half_sub(t.i, f1.i)};
+}
+
+// Hermitian merge before the half-length inverse transform, kiss_fftr.c:131-153.
+template <class A>
+__device__ __forceinline__ Cx<A> merge_dc(Cx<A> x0, Cx<A> xn)
+{
+    typedef Ops<A> O;
+    return O::fixdiv(Cx<A>{O::add(x0.r, xn.r), O::sub(x0.r, xn.r)}, 2);
+}
+
+// Z[k] and Z[nc-k] (0 < k <= nc/2) from X[k], X[nc-k] and tw = stw[k-1].
+template <class A>
+__device__ __forceinline__ void merge_k(Cx<A> xk, Cx<A> xnk, Cx<A> tw, Cx<A> &zk, Cx<A> &znk)
+{
+    typedef Ops<A> O;
+    Cx<A> a = O::fixdiv(xk, 2);
+    Cx<A> b = O::fixdiv(Cx<A>{xnk.r, O::neg(xnk.i)}, 2);
+    const Cx<A> e = O::cadd(a, b), d = O::csub(a, b);
+    const Cx<A> o = O::cmul(d, tw);
+    Cx<A> lo = O::csub(e, o);
+    lo.i = O::neg(lo.i);
+    zk = O::cadd(e, o);
+    znk = lo;
+}
+
 template <class A>
 __device__ __forceinline__ void r2c_epilogue(const LineCtx<A> &cx, const typename A::store2 *z,
                                              typename A::store2 *__restrict__ out,
@@ -152,26 +198,20 @@ __device__ __forceinline__ void r2c_epilogue(const LineCtx<A> &cx, const typenam
             continue;
         const long long l = l0 + line;
         const long long base = (l / io.nlo) * io.out_hi + (l % io.nlo) * io.out_lo;
+        C xa, xb;
         if (k == 0) {
-            C dc = O::fixdiv(O::load(z[cx.idx(line, 0)]), 2);
-            out[base] = O::pack(C{O::add(dc.r, dc.i), O::zero().r});
-            out[base + (long long)nc * io.out_el] = O::pack(C{O::sub(dc.r, dc.i), O::zero().r});
+            split_dc<A>(O::load(z[cx.idx(line, 0)]), xa, xb);
+            out[base] = O::pack(xa);
+            out[base + (long long)nc * io.out_el] = O::pack(xb);
             continue;
         }
-        C a = O::load(z[cx.idx(line, k)]);
-        C bz = O::load(z[cx.idx(line, nc - k)]);
-        C b{bz.r, O::neg(bz.i)};
-        a = O::fixdiv(a, 2);
-        b = O::fixdiv(b, 2);
-        C f1 = O::cadd(a, b), f2 = O::csub(a, b);
-        C t = O::cmul(f2, O::load(stw[k - 1]));
+        split_k<A>(O::load(z[cx.idx(line, k)]), O::load(z[cx.idx(line, nc - k)]), O::load(stw[k - 1]), xa, xb);
         if (2 * k != nc)
-            out[base + (long long)k * io.out_el] = O::pack(C{O::half_add(f1.r, t.r), O::half_add(f1.i, t.i)});
-        out[base + (long long)(nc - k) * io.out_el] = O::pack(C{O::half_sub(f1.r, t.r), O::half_sub(t.i, f1.i)});
+            out[base + (long long)k * io.out_el] = O::pack(xa);
+        out[base + (long long)(nc - k) * io.out_el] = O::pack(xb);
     }
 }
 
-// Hermitian merge before the half-length inverse transform, kiss_fftr.c:131-153.
 template <class A>
 __device__ __forceinline__ void c2r_prologue(const LineCtx<A> &cx, typename A::store2 *z,
                                              const typename A::store2 *__restrict__ in,
@@ -189,23 +229,74 @@ __device__ __forceinline__ void c2r_prologue(const LineCtx<A> &cx, typename A::s
         const long long l = l0 + line;
         const long long base = (l / io.nlo) * io.in_hi + (l % io.nlo) * io.in_lo;
         if (k == 0) {
-            C x0 = O::load(in[base]), xn = O::load(in[base + (long long)nc * io.in_el]);
-            C d{O::add(x0.r, xn.r), O::sub(x0.r, xn.r)};
-            z[cx.idx(line, 0)] = O::pack(O::fixdiv(d, 2));
+            z[cx.idx(line, 0)] = O::pack(merge_dc<A>(O::load(in[base]), O::load(in[base + (long long)nc * io.in_el])));
             continue;
         }
-        C a = O::load(in[base + (long long)k * io.in_el]);
-        C bz = O::load(in[base + (long long)(nc - k) * io.in_el]);
-        C b{bz.r, O::neg(bz.i)};
-        a = O::fixdiv(a, 2);
-        b = O::fixdiv(b, 2);
-        C e = O::cadd(a, b), d = O::csub(a, b);
-        C o = O::cmul(d, O::load(stw[k - 1]));
-        C lo = O::csub(e, o);
-        lo.i = O::neg(lo.i);
+        C za, zb;
+        merge_k<A>(O::load(in[base + (long long)k * io.in_el]), O::load(in[base + (long long)(nc - k) * io.in_el]),
+                   O::load(stw[k - 1]), za, zb);
         if (2 * k != nc)
-            z[cx.idx(line, k)] = O::pack(O::cadd(e, o));
-        z[cx.idx(line, nc - k)] = O::pack(lo);
+            z[cx.idx(line, k)] = O::pack(za);
+        z[cx.idx(line, nc - k)] = O::pack(zb);
+    }
+}
+
+// Standalone split / merge passes (HBM -> HBM) for half lengths longer than one LDS line:
+// the half-length complex transform runs as a multi-pass transform into a temporary and
+// these kernels apply kiss_fftr.c:88-116 (split) or kiss_fftr.c:131-153 (merge) to it.
+// Thread k of line l handles the pair (k, nc-k), k in [0, nc/2]; grid.y strides lines.
+// Distances are in complex values; z lines are nc apart.
+template <class A>
+__global__ void __launch_bounds__(256) k_real_split(const typename A::store2 *__restrict__ z,
+                                                    typename A::store2 *__restrict__ out,
+                                                    const typename A::store2 *__restrict__ stw, int nc,
+                                                    long long out_hi, long long nlines)
+{
+    typedef Ops<A> O;
+    typedef Cx<A> C;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > nc / 2)
+        return;
+    for (long long l = blockIdx.y; l < nlines; l += gridDim.y) {
+        const typename A::store2 *zl = z + l * nc;
+        typename A::store2 *ol = out + l * out_hi;
+        C xa, xb;
+        if (k == 0) {
+            split_dc<A>(O::load(zl[0]), xa, xb);
+            ol[0] = O::pack(xa);
+            ol[nc] = O::pack(xb);
+            continue;
+        }
+        split_k<A>(O::load(zl[k]), O::load(zl[nc - k]), O::load(stw[k - 1]), xa, xb);
+        if (2 * k != nc)
+            ol[k] = O::pack(xa);
+        ol[nc - k] = O::pack(xb);
+    }
+}
+
+template <class A>
+__global__ void __launch_bounds__(256) k_real_merge(const typename A::store2 *__restrict__ in,
+                                                    typename A::store2 *__restrict__ z,
+                                                    const typename A::store2 *__restrict__ stw, int nc,
+                                                    long long in_hi, long long nlines)
+{
+    typedef Ops<A> O;
+    typedef Cx<A> C;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > nc / 2)
+        return;
+    for (long long l = blockIdx.y; l < nlines; l += gridDim.y) {
+        const typename A::store2 *il = in + l * in_hi;
+        typename A::store2 *zl = z + l * nc;
+        if (k == 0) {
+            zl[0] = O::pack(merge_dc<A>(O::load(il[0]), O::load(il[nc])));
+            continue;
+        }
+        C za, zb;
+        merge_k<A>(O::load(il[k]), O::load(il[nc - k]), O::load(stw[k - 1]), za, zb);
+        if (2 * k != nc)
+            zl[k] = O::pack(za);
+        zl[nc - k] = O::pack(zb);
     }
 }
 

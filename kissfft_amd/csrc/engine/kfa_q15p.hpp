@@ -1,0 +1,187 @@
+// kfa_q15p.hpp -- Q15 butterflies on packed complex words.
+//
+// A Q15 complex is one 32-bit word: real part in the low 16 bits, imaginary part in the
+// high 16 bits -- the in-memory int16 pair (kiss_fft_cpx, kiss_fft.h:87-90) loaded as is.
+// Each reference operation maps onto CDNA packed-integer instructions with the same bits:
+//   C_ADD / C_SUB with truncation to int16   -> v_pk_add_u16 / v_pk_sub_u16 (wrap per half)
+//   C_MUL, one sround per component          -> v_dot2_i32_i16 (a.r*b.r - a.i*b.i + 2^14 in
+//                                               one instruction, no intermediate rounding),
+//                                               then bits 15..30 of each sum repacked
+//   C_FIXDIV / C_MULBYSCALAR / S_MUL         -> the same dot2 form with one zero lane
+//   HALF_OF of a stored scalar               -> v_pk_ashrrev_i16
+//   multiplication by -i / +i                -> a 16-bit rotate plus v_pk_mul_lo_u16 by (1,-1)
+// Operation order per butterfly is the reference's (kiss_fft.c:15-189, macros of
+// _kiss_fft_guts.h:43-124): sums of stored values may be regrouped because int16
+// truncation is a ring homomorphism; every rounded product sees exactly the int16 operands
+// the reference rounds.
+//
+// Twiddles are stored pre-arranged: entry w becomes two words (w.r, -w.i) and (w.i, w.r),
+// so the real and imaginary dot products read them directly (|w.i| <= 32767, so -w.i fits).
+#pragma once
+
+#include "kfa_arith.hpp"
+
+namespace kfa {
+
+struct Q15P {
+    typedef q15x2 store2;
+    typedef unsigned reg;
+    static constexpr bool fixed = true;
+    static constexpr bool packed = true;
+};
+
+template <>
+struct Cx<Q15P> {
+    unsigned w;
+};
+
+namespace q15p {
+
+typedef short v2s __attribute__((ext_vector_type(2)));
+typedef unsigned u32;
+
+__device__ __forceinline__ v2s as_v(u32 w) { return __builtin_bit_cast(v2s, w); }
+__device__ __forceinline__ u32 as_u(v2s v) { return __builtin_bit_cast(u32, v); }
+
+__device__ __forceinline__ u32 add(u32 a, u32 b) { return as_u(as_v(a) + as_v(b)); }
+__device__ __forceinline__ u32 sub(u32 a, u32 b) { return as_u(as_v(a) - as_v(b)); }
+__device__ __forceinline__ u32 half(u32 a) { return as_u(as_v(a) >> (v2s){1, 1}); }
+__device__ __forceinline__ u32 mul_lo(u32 a, u32 m) { return as_u(as_v(a) * as_v(m)); }
+// (a.i, a.r)
+__device__ __forceinline__ u32 swap(u32 a) { return (a >> 16) | (a << 16); }
+// (x.lo, y.hi)
+__device__ __forceinline__ u32 lohi(u32 x, u32 y) { return (x & 0xffffu) | (y & 0xffff0000u); }
+
+// bits 15..30 of two sums, packed (sround of each component, truncated to int16)
+__device__ __forceinline__ u32 round_pack(int re, int im) { return lohi((u32)(re >> 15), (u32)im << 1); }
+
+__device__ __forceinline__ int dot(u32 a, u32 b)
+{
+    return __builtin_amdgcn_sdot2(as_v(a), as_v(b), 1 << 14, false);
+}
+
+// complex product with a pre-arranged twiddle pair (w.r, -w.i), (w.i, w.r)
+__device__ __forceinline__ u32 cmul(u32 a, u32 wr, u32 wi) { return round_pack(dot(a, wr), dot(a, wi)); }
+
+// both components times the scalar s: pass (s, 0) and (0, s)
+__device__ __forceinline__ u32 scale(u32 a, u32 s_lo, u32 s_hi) { return round_pack(dot(a, s_lo), dot(a, s_hi)); }
+
+struct Scal {
+    u32 lo, hi;  // (s, 0) and (0, s)
+};
+__device__ __forceinline__ Scal scal(int s) { return Scal{(u32)s & 0xffffu, (u32)s << 16}; }
+__device__ __forceinline__ u32 scale(u32 a, Scal s) { return scale(a, s.lo, s.hi); }
+
+// -i * z = (z.i, -z.r) and +i * z = (-z.i, z.r)
+__device__ __forceinline__ u32 mul_mi(u32 z) { return mul_lo(swap(z), 0xffff0001u); }
+__device__ __forceinline__ u32 mul_pi(u32 z) { return mul_lo(swap(z), 0x0001ffffu); }
+
+__device__ __forceinline__ int lo16(u32 w) { return (int)(short)(w & 0xffffu); }
+
+// C_FIXDIV(c, p): K = floor(32767 / p)
+template <int P>
+__device__ __forceinline__ u32 fixdiv(u32 c)
+{
+    constexpr int K = 32767 / P;
+    return scale(c, (u32)K, (u32)K << 16);
+}
+
+// kiss_fft.c:15-36
+__device__ __forceinline__ void bfly2(u32 *v, const uint2 *tw)
+{
+    u32 a = fixdiv<2>(v[0]), b = fixdiv<2>(v[1]);
+    u32 t = cmul(b, tw[0].x, tw[0].y);
+    v[1] = sub(a, t);
+    v[0] = add(a, t);
+}
+
+// kiss_fft.c:38-84
+template <bool INV>
+__device__ __forceinline__ void bfly4(u32 *v, const uint2 *tw)
+{
+    u32 x0 = fixdiv<4>(v[0]), x1 = fixdiv<4>(v[1]), x2 = fixdiv<4>(v[2]), x3 = fixdiv<4>(v[3]);
+    u32 s0 = cmul(x1, tw[0].x, tw[0].y);
+    u32 s1 = cmul(x2, tw[1].x, tw[1].y);
+    u32 s2 = cmul(x3, tw[2].x, tw[2].y);
+    u32 s5 = sub(x0, s1);
+    x0 = add(x0, s1);
+    u32 s3 = add(s0, s2);
+    u32 s4 = sub(s0, s2);
+    v[2] = sub(x0, s3);
+    v[0] = add(x0, s3);
+    u32 s4s = swap(s4);
+    u32 A = add(s5, s4s);  // (s5.r + s4.i, s5.i + s4.r)
+    u32 B = sub(s5, s4s);  // (s5.r - s4.i, s5.i - s4.r)
+    if (INV) {
+        v[1] = lohi(B, A);
+        v[3] = lohi(A, B);
+    } else {
+        v[1] = lohi(A, B);
+        v[3] = lohi(B, A);
+    }
+}
+
+// kiss_fft.c:86-128; e = (epi3.i, 0), (0, epi3.i)
+__device__ __forceinline__ void bfly3(u32 *v, const uint2 *tw, Scal e)
+{
+    u32 x0 = fixdiv<3>(v[0]), x1 = fixdiv<3>(v[1]), x2 = fixdiv<3>(v[2]);
+    u32 s1 = cmul(x1, tw[0].x, tw[0].y);
+    u32 s2 = cmul(x2, tw[1].x, tw[1].y);
+    u32 s3 = add(s1, s2);
+    u32 s0 = sub(s1, s2);
+    u32 o1 = sub(x0, half(s3));
+    s0 = scale(s0, e);
+    x0 = add(x0, s3);
+    u32 s0s = swap(s0);
+    u32 A = add(o1, s0s);  // (o1.r + s0.i, o1.i + s0.r)
+    u32 B = sub(o1, s0s);  // (o1.r - s0.i, o1.i - s0.r)
+    v[0] = x0;
+    v[1] = lohi(B, A);
+    v[2] = lohi(A, B);
+}
+
+// kiss_fft.c:130-189
+struct Y5 {
+    Scal yar, yai, ybr, ybi;
+};
+
+__device__ __forceinline__ void bfly5(u32 *v, const uint2 *tw, const Y5 &y)
+{
+    const u32 x0 = fixdiv<5>(v[0]);
+    u32 c1 = cmul(fixdiv<5>(v[1]), tw[0].x, tw[0].y);
+    u32 c2 = cmul(fixdiv<5>(v[2]), tw[1].x, tw[1].y);
+    u32 c3 = cmul(fixdiv<5>(v[3]), tw[2].x, tw[2].y);
+    u32 c4 = cmul(fixdiv<5>(v[4]), tw[3].x, tw[3].y);
+    u32 c7 = add(c1, c4), c10 = sub(c1, c4);
+    u32 c8 = add(c2, c3), c9 = sub(c2, c3);
+    u32 c5 = add(add(x0, scale(c7, y.yar)), scale(c8, y.ybr));
+    u32 c6 = mul_mi(add(scale(c10, y.yai), scale(c9, y.ybi)));
+    u32 c11 = add(add(x0, scale(c7, y.ybr)), scale(c8, y.yar));
+    u32 c12 = mul_pi(sub(scale(c10, y.ybi), scale(c9, y.yai)));
+    v[0] = add(x0, add(c7, c8));
+    v[1] = sub(c5, c6);
+    v[4] = add(c5, c6);
+    v[2] = add(c11, c12);
+    v[3] = sub(c11, c12);
+}
+
+}  // namespace q15p
+
+template <class A>
+struct IsPacked {
+    static constexpr bool value = false;
+};
+template <>
+struct IsPacked<Q15P> {
+    static constexpr bool value = true;
+};
+
+template <>
+struct Ops<Q15P> {
+    typedef Cx<Q15P> C;
+    __device__ __forceinline__ static C load(const q15x2 &s) { return C{__builtin_bit_cast(unsigned, s)}; }
+    __device__ __forceinline__ static q15x2 pack(C c) { return __builtin_bit_cast(q15x2, c.w); }
+    __device__ __forceinline__ static C zero() { return C{0u}; }
+};
+
+}  // namespace kfa

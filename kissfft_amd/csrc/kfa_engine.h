@@ -66,6 +66,30 @@ KFA_HIDDEN void kfa_cleanup(void);
 KFA_HIDDEN const char *kfa_last_error(void);
 KFA_HIDDEN int kfa_device_available(void);
 
+/* ---- request server (kfa_serve.hip) ---------------------------------------------------- */
+enum { KFA_SERVE_C2C_FWD = 0, KFA_SERVE_C2C_INV = 1, KFA_SERVE_R2C = 2, KFA_SERVE_C2R = 3 };
+
+typedef struct {
+    long long submitted, completed, failed;
+    long long launches, grouped_requests;
+    long long bytes;          /* algorithmic bytes of completed requests */
+    long long budget_bytes, in_use_bytes;
+    double p50_s, p99_s, max_s, mean_s;
+    double span_s;            /* first submit to last completion */
+} kfa_server_stats;
+
+/* budget_bytes <= 0: 80% of the HBM free at creation.  nstreams <= 0: 4.  max_group <= 0: 64. */
+KFA_HIDDEN void *kfa_server_create(int type, long long budget_bytes, int nstreams, int max_group);
+/* cp for complex kinds, rp for real kinds; nfft is the transform length (real length for
+ * real kinds).  Returns a ticket > 0 or a negative code. */
+KFA_HIDDEN long long kfa_server_submit(void *srv, int kind, const kfa_cplan *cp, const kfa_rplan *rp, int nfft,
+                                       const void *in, void *out, long long howmany);
+KFA_HIDDEN int kfa_server_wait(void *srv, long long ticket, double *latency_s);
+KFA_HIDDEN int kfa_server_drain(void *srv);
+KFA_HIDDEN int kfa_server_get_stats(void *srv, kfa_server_stats *st);
+KFA_HIDDEN void kfa_server_reset_stats(void *srv);
+KFA_HIDDEN void kfa_server_destroy(void *srv);
+
 /* Counters for tests and the serving layer. */
 typedef struct {
     long long plans_built;

@@ -65,6 +65,38 @@ def test_q15_real_bit_exact_vs_oracle():
         np.testing.assert_array_equal(c2r(lib, n, X), O.fftri(X, n))
 
 
+@pytest.mark.parametrize("datatype,n", [("int16_t", 65536), ("int16_t", 40000), ("int32_t", 32768),
+                                         ("int32_t", 20000)])
+def test_fixed_real_multipass_bit_exact_vs_oracle(datatype, n):
+    # half length beyond one LDS line: four-step complex pass + separate split/merge pass
+    lib = kissfft_amd.load(datatype)
+    O = Oracle(datatype)
+    rng = np.random.default_rng(32)
+    info = np.iinfo(lib.scalar)
+    x = rng.integers(info.min, info.max, (3, n), endpoint=True).astype(lib.scalar)
+    say("case real multipass", datatype, n)
+    np.testing.assert_array_equal(r2c(lib, n, x), O.fftr(x))
+    X = rng.integers(info.min, info.max, (3, n // 2 + 1, 2), endpoint=True).astype(lib.scalar)
+    np.testing.assert_array_equal(c2r(lib, n, X), O.fftri(X, n))
+
+
+@pytest.mark.parametrize("n,howmany", [(32768, 8), (65536, 4), (1 << 20, 2), (3 * 5 * 7 * 1024, 2)])
+def test_float_real_multipass(n, howmany):
+    lib = kissfft_amd.load("float")
+    x = W.two_tone(5, n, howmany)
+    say("case float real multipass", n)
+    X = r2c(lib, n, x)
+    want = np.fft.rfft(x.astype(np.float64), axis=-1)
+    err = rel_l2(cplx(X), want)
+    say("r2c rel-L2", err)
+    assert err <= ABS_FLOAT_BOUND
+    Xin = pairs(want.astype(np.complex64))
+    y = c2r(lib, n, Xin)
+    err = rel_l2(y, np.fft.irfft(cplx(Xin), n=n, axis=-1) * n)
+    say("c2r rel-L2", err)
+    assert err <= ABS_FLOAT_BOUND
+
+
 @pytest.mark.parametrize("n", [4096, 4000])
 def test_float_real_workload_sample(n, golden_float):
     lib = kissfft_amd.load("float")

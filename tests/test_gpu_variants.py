@@ -56,7 +56,7 @@ def test_registry_parsed():
 
 @pytest.mark.parametrize("typ,kind,n,name", CASES, ids=[f"{c[3]}-{c[1]}" for c in CASES])
 def test_forced_variant(typ, kind, n, name, monkeypatch):
-    datatype = {"F32": "float", "Q15": "int16_t"}[typ]
+    datatype = {"F32": "float", "Q15": "int16_t", "Q15P": "int16_t"}[typ]
     lib = kissfft_amd.load(datatype)
     lib.L.kiss_fft_cleanup()
     monkeypatch.setenv("KISSFFT_AMD_VARIANT", name)
