@@ -139,6 +139,105 @@ def other_configs(lib, stream, dev) -> dict:
     return out
 
 
+def serving(lib, dev, count: int, rate: float) -> dict:
+    """The serving stream: `count` seeded requests (c2c / r2c / inverse, n in {256, 1000,
+    1009, 3072, 65536, 1048576}, batch log-uniform in [1, 4096]) through one request server.
+
+    open loop:  Poisson arrivals at `rate` per second, submitted at their arrival times;
+                p50 / p99 latency (submit to completion) under that offered load.
+    saturated:  the same requests all submitted at once; throughput = algorithmic bytes /
+                (first submit to last completion).
+    per-call:   the same requests as a loop of single-transform kiss_fft / kiss_fftr /
+                kiss_fftri calls (one call per transform, device pointers, synchronous), the
+                drop-in API used one transform at a time.
+    Inputs (uniform [-1, 1)) and outputs are resident in HBM before timing starts; every
+    request of one (kind, n) reads the same input buffer (read-only), each request writes
+    its own output buffer."""
+    import torch
+
+    from kissfft_amd import workloads as W
+
+    reqs = W.serving_stream(count, rate)
+    kinds = {"c2c": 0, "c2c_inv": 1, "r2c": 2, "c2r": 3}
+    inputs = {}
+    for r in reqs:
+        key = (r["kind"], r["n"])
+        inputs[key] = max(inputs.get(key, 0), W.request_bytes(r["kind"], r["n"], r["batch"])[0])
+    ibufs = {k: (torch.rand(v // 4, device=dev) * 2 - 1) for k, v in inputs.items()}
+    obufs = [torch.empty(W.request_bytes(r["kind"], r["n"], r["batch"])[1] // 4, device=dev) for r in reqs]
+    torch.cuda.synchronize()
+    total_bytes = sum(sum(W.request_bytes(r["kind"], r["n"], r["batch"])) for r in reqs)
+    transforms = sum(r["batch"] for r in reqs)
+
+    srv = lib.server()
+    # warm-up: plans, twiddles and variant choice of every (kind, n), one transform each
+    for key in inputs:
+        warm_out = torch.empty(W.request_bytes(key[0], key[1], 1)[1] // 4, device=dev)
+        srv.wait(srv.submit(kinds[key[0]], key[1], ibufs[key], warm_out, 1))
+    srv.reset_stats()
+
+    def run(open_loop: bool):
+        t0 = time.perf_counter()
+        tickets = []
+        for r, ob in zip(reqs, obufs):
+            if open_loop:
+                while time.perf_counter() - t0 < r["t"]:
+                    pass
+            tickets.append(srv.submit(kinds[r["kind"]], r["n"], ibufs[(r["kind"], r["n"])], ob, r["batch"]))
+        for t in tickets:
+            srv.wait(t)
+        st = srv.stats()
+        srv.reset_stats()
+        return st
+
+    ol = run(True)
+    sat = run(False)
+    sample = [0, len(reqs) // 2, len(reqs) - 1]
+    served = {i: obufs[i][: W.request_bytes(reqs[i]["kind"], reqs[i]["n"], 1)[1] // 4].cpu() for i in sample}
+
+    cfgs = {}
+    for key in inputs:
+        kind, n = key
+        cfgs[key] = (lib.fft_alloc(n, kind == "c2c_inv") if kind.startswith("c2c")
+                     else lib.fftr_alloc(n, kind == "c2r"))
+    L = lib.L
+    calls = {"c2c": L.kiss_fft, "c2c_inv": L.kiss_fft, "r2c": L.kiss_fftr, "c2r": L.kiss_fftri}
+    t0 = time.perf_counter()
+    for r, ob in zip(reqs, obufs):
+        key = (r["kind"], r["n"])
+        ib, ob1 = W.request_bytes(r["kind"], r["n"], 1)
+        fn, cfg = calls[r["kind"]], cfgs[key]
+        ip, op = ibufs[key].data_ptr(), ob.data_ptr()
+        for b in range(r["batch"]):
+            fn(cfg, ip + b * ib, op + b * ob1)
+    torch.cuda.synchronize()
+    per_call_s = time.perf_counter() - t0
+    same = all(torch.equal(served[i], obufs[i][: served[i].numel()].cpu()) for i in sample)
+    srv.close()
+    del ibufs, obufs
+    torch.cuda.empty_cache()
+
+    def ms(v):
+        return round(v * 1e3, 3)
+
+    sat_gbps = total_bytes / sat["span_s"] / 1e9
+    pc_gbps = total_bytes / per_call_s / 1e9
+    return {
+        "requests": len(reqs), "transforms": transforms, "alg_bytes": total_bytes,
+        "sizes": list(W.SERVE_SIZES), "kinds": "c2c, r2c, inverse (c2r; at odd n 1009: r2c->c2c, inverse->c2c inverse)",
+        "open_loop": {"rate_per_s": rate, "offered_GBps": round(total_bytes / reqs[-1]["t"] / 1e9, 1),
+                      "p50_ms": ms(ol["p50_s"]), "p99_ms": ms(ol["p99_s"]), "max_ms": ms(ol["max_s"]),
+                      "GBps": round(total_bytes / ol["span_s"] / 1e9, 1), "launches": ol["launches"]},
+        "saturated": {"GBps": round(sat_gbps, 1), "transforms_per_s": round(transforms / sat["span_s"], 1),
+                      "span_ms": ms(sat["span_s"]), "p50_ms": ms(sat["p50_s"]), "p99_ms": ms(sat["p99_s"]),
+                      "launches": sat["launches"]},
+        "per_call": {"GBps": round(pc_gbps, 2), "transforms_per_s": round(transforms / per_call_s, 1),
+                     "seconds": round(per_call_s, 3)},
+        "speedup_vs_per_call": round(sat_gbps / pc_gbps, 2),
+        "served_equals_per_call": bool(same),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,6 +246,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-other-configs", action="store_true",
                     help="skip the secondary workloads reported under other_configs")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-stream measurement")
+    ap.add_argument("--serve-requests", type=int, default=192)
+    ap.add_argument("--serve-rate", type=float, default=2000.0, help="open-loop arrivals per second")
     ap.add_argument("--pmc-run", action="store_true",
                     help="for rocprofv3 --pmc: one plain copy of a known size (the counter scale), "
                          "then one forward and one inverse N=4096 launch, and exit")
@@ -240,6 +342,11 @@ def main() -> int:
             per_kernel[f"{name}_{n}"] = {"us": round(s * 1e6, 1), "GBps": round(alg_bytes(n) * BATCH / s / 1e9, 1)}
 
     others = {} if args.no_other_configs else other_configs(lib, stream, dev)
+    serve = None
+    if not args.no_serving:
+        del bufs, x, X, y
+        torch.cuda.empty_cache()
+        serve = serving(lib, dev, args.serve_requests, args.serve_rate)
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic_r2c_4096.json")
@@ -276,11 +383,12 @@ def main() -> int:
         },
         "pct_hbm_peak": round(100 * value / HBM_PEAK_GBPS, 2),
         "other_configs": others,
+        "serving": serve,
         "transforms_per_s": round(transforms_per_s, 1),
         "per_kernel": per_kernel,
         "roofline": {
             "bound": "hbm",
-            "kernel": "forward real N=4096 (k_lines<F32>, fused split epilogue)",
+            "kernel": "forward real N=4096 (k_fast<F32, FK_R2C>: LDS-resident Stockham + real split)",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",

@@ -36,7 +36,7 @@ DATATYPES = {
     "int32_t": ["-DFIXED_POINT=32"],
 }
 HOST_SOURCES = ["kf_core.c", "kf_real.c", "kf_nd.c", "kf_ndr.c", "kf_cache.c", "kf_serve.c"]
-ENGINE_SOURCES = ["kfa_engine.hip", "kfa_fast_reg.hip", "kfa_serve.hip"]
+ENGINE_SOURCES = ["kfa_engine.hip", "kfa_fast_reg.hip", "kfa_tile_reg.hip", "kfa_serve.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CC = os.environ.get("CC", "gcc")

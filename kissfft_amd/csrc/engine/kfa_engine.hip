@@ -14,7 +14,9 @@
 // bit-exact).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -30,6 +32,7 @@
 #include "kfa_fast_reg.hpp"
 #include "kfa_internal.hpp"
 #include "kfa_lines.hpp"
+#include "kfa_tile.hpp"
 
 namespace kfa {
 
@@ -57,6 +60,50 @@ int fail(int code, const char *fmt, ...)
 std::atomic<long long> g_plans_built{0}, g_plan_hits{0}, g_launches{0}, g_h2d{0}, g_d2h{0};
 
 size_t cpx_bytes(int type) { return type == KFA_Q15 ? 4 : 8; }
+
+// ------------------------------------------------------------------------------------
+// scratch memory: one stream-ordered pool per device that keeps what it has mapped
+// (release threshold: unlimited), so repeated large temporaries -- four-step passes,
+// staging copies, the request server's working set -- are not mapped again on every
+// call.  kfa_cleanup trims it.
+// ------------------------------------------------------------------------------------
+static std::mutex g_pool_mu;
+static std::map<int, hipMemPool_t> g_pools;
+
+hipError_t scratch_alloc(void **p, size_t bytes, hipStream_t s)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess)
+        return e;
+    hipMemPool_t pool;
+    {
+        std::lock_guard<std::mutex> g(g_pool_mu);
+        auto it = g_pools.find(dev);
+        if (it == g_pools.end()) {
+            hipMemPoolProps props = {};
+            props.allocType = hipMemAllocationTypePinned;
+            props.handleTypes = hipMemHandleTypeNone;
+            props.location.type = hipMemLocationTypeDevice;
+            props.location.id = dev;
+            if ((e = hipMemPoolCreate(&pool, &props)) != hipSuccess)
+                return e;
+            uint64_t keep = ~0ULL;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+            g_pools[dev] = pool;
+        } else {
+            pool = it->second;
+        }
+    }
+    return hipMallocFromPoolAsync(p, bytes ? bytes : 1, pool, s);
+}
+
+static void scratch_trim()
+{
+    std::lock_guard<std::mutex> g(g_pool_mu);
+    for (auto &kv : g_pools)
+        (void)hipMemPoolTrimTo(kv.second, 0);
+}
 
 // largest line kept whole in LDS (two ping-pong buffers must fit 160 KiB)
 int line_cap(int type) { return type == KFA_Q15 ? 16384 : 8192; }
@@ -226,15 +273,15 @@ static bool env_flag(const char *name, bool dflt)
 // Gathers the stage tables of schedule `rad` from the full-length table: row u of stage s
 // holds table[r*u*P_s] for r = 1..p-1, padded to a multiple of `pad` entries; then
 // table[P_s*m_s] and table[2*P_s*m_s], padded likewise (the layout of Radices<G, ...>).
-static int make_stage_table(const DevPlan &P, const std::vector<int> &rad, int pad, int fmt, void **d_out)
+static int make_stage_table_raw(const unsigned char *h_tw, int type, long long n, const std::vector<int> &rad,
+                                int pad, int fmt, void **d_out)
 {
     // fmt 1: a Q15 entry w becomes the word pair (w.r, -w.i), (w.i, w.r) (8 bytes)
-    const size_t cb = fmt == 1 ? 8 : cpx_bytes(P.type);
-    const long long n = P.n;
+    const size_t cb = fmt == 1 ? 8 : cpx_bytes(type);
     std::vector<unsigned char> h;
     long long m = 1;
     auto put = [&](long long idx) {
-        const unsigned char *src = P.h_tw.data() + cpx_bytes(P.type) * (size_t)(idx % n);
+        const unsigned char *src = h_tw + cpx_bytes(type) * (size_t)(idx % n);
         if (fmt == 1) {
             int16_t wr, wi;
             memcpy(&wr, src, 2);
@@ -272,6 +319,196 @@ static int make_stage_table(const DevPlan &P, const std::vector<int> &rad, int p
     KFA_HIP(hipMalloc(d_out, h.size()));
     KFA_HIP(hipMemcpy(*d_out, h.data(), h.size(), hipMemcpyHostToDevice));
     return 0;
+}
+
+static int make_stage_table(const DevPlan &P, const std::vector<int> &rad, int pad, int fmt, void **d_out)
+{
+    return make_stage_table_raw(P.h_tw.data(), P.type, P.n, rad, pad, fmt, d_out);
+}
+
+// Float twiddles of length n, the host library's expression (kiss_fft.c:361-367): phase
+// -2*pi*i/n in FP64 (negated for the inverse), cos / sin rounded once to float.
+static std::vector<unsigned char> float_twiddles(long long n, int inverse, long long count, long long step)
+{
+    const double pi = 3.14159265358979323846264338327;
+    std::vector<unsigned char> h((size_t)count * 8);
+    for (long long i = 0; i < count; ++i) {
+        double phase = -2 * pi * (double)(i * step % n) / (double)n;
+        if (inverse)
+            phase *= -1;
+        const float v[2] = {(float)cos(phase), (float)sin(phase)};
+        memcpy(h.data() + 8 * i, v, 8);
+    }
+    return h;
+}
+
+// ------------------------------------------------------------------------------------
+// tile variants (float): choice per (length, direction, mapping), four-step twiddles
+// ------------------------------------------------------------------------------------
+struct TileChoice {
+    const TileVariant *v = nullptr;
+    void *d_stw = nullptr;
+};
+
+struct TileCache {
+    std::mutex mu;
+    std::map<std::tuple<int, int, int, int>, TileChoice> choice;            // dev, n, inv, mode
+    std::map<std::tuple<int, long long, int>, std::pair<void *, void *>> fs; // dev, N, inv -> hi, lo
+};
+
+static TileCache &tile_cache()
+{
+    static TileCache *c = new TileCache();
+    return *c;
+}
+
+// The tile variant for (n, inverse, mode) on the current device, or v == nullptr.
+static int tile_choice(int n, int inverse, int mode, const TileChoice **out)
+{
+    int dev;
+    int rc = current_device(&dev);
+    if (rc)
+        return rc;
+    auto &C = tile_cache();
+    std::lock_guard<std::mutex> g(C.mu);
+    const auto key = std::make_tuple(dev, n, inverse, mode);
+    auto it = C.choice.find(key);
+    if (it != C.choice.end()) {
+        *out = &it->second;
+        return 0;
+    }
+    TileChoice tc;
+    if (env_flag("KISSFFT_AMD_FAST", true)) {
+        const char *force = getenv("KISSFFT_AMD_TILE");
+        for (const TileVariant &v : tile_registry()) {
+            if (v.n != n || v.inverse != inverse || v.mode != mode)
+                continue;
+            if (!tc.v || (force && strcmp(force, v.name) == 0))
+                tc.v = &v;
+        }
+    }
+    if (tc.v) {
+        const std::vector<unsigned char> tw = float_twiddles(n, inverse, n, 1);
+        rc = make_stage_table_raw(tw.data(), KFA_F32, n, tc.v->radices, tc.v->tw_pad, 0, &tc.d_stw);
+        if (rc)
+            return rc;
+        if (env_flag("KISSFFT_AMD_VERBOSE", false))
+            fprintf(stderr, "[kissfft_amd] tile n=%d inverse=%d mode=%d: %s\n", n, inverse, mode, tc.v->name);
+    }
+    auto &slot = C.choice[key];
+    slot = tc;
+    *out = &slot;
+    return 0;
+}
+
+// W^(q*1024) for q <= N/1024 and W^r for r < 1024, W = exp(-+2*pi*i/N)
+static int fourstep_twiddles(long long N, int inverse, const void **hi, const void **lo)
+{
+    int dev;
+    int rc = current_device(&dev);
+    if (rc)
+        return rc;
+    auto &C = tile_cache();
+    std::lock_guard<std::mutex> g(C.mu);
+    const auto key = std::make_tuple(dev, N, inverse);
+    auto it = C.fs.find(key);
+    if (it == C.fs.end()) {
+        const std::vector<unsigned char> h = float_twiddles(N, inverse, N / 1024 + 1, 1024);
+        const std::vector<unsigned char> l = float_twiddles(N, inverse, 1024, 1);
+        void *dh = nullptr, *dl = nullptr;
+        KFA_HIP(hipMalloc(&dh, h.size()));
+        KFA_HIP(hipMalloc(&dl, l.size()));
+        KFA_HIP(hipMemcpy(dh, h.data(), h.size(), hipMemcpyHostToDevice));
+        KFA_HIP(hipMemcpy(dl, l.data(), l.size(), hipMemcpyHostToDevice));
+        it = C.fs.emplace(key, std::make_pair(dh, dl)).first;
+    }
+    *hi = it->second.first;
+    *lo = it->second.second;
+    return 0;
+}
+
+static int launch_tile(const TileChoice &tc, TileArgs a, hipStream_t s)
+{
+    a.stw = tc.d_stw;
+    const long long tiles = (a.nlines + tc.v->TL - 1) / tc.v->TL;
+    // tiles whose column segments are narrower than a 128-byte line share lines
+    const long long seg = (long long)tc.v->TL * 8;
+    a.xcd_group = (a.in_lo == 1 && seg < 128) ? (int)(128 / seg) : 1;
+    if (tiles > 0x7fffffffLL)
+        return fail(-2, "too many lines (%lld) for one launch", a.nlines);
+    ++g_launches;
+    return tc.v->launch(a, tiles, s) ? fail(-3, "tile kernel launch failed") : 0;
+}
+
+// Float four-step (Bailey) for n = N1*N2 with both halves registered as tile variants:
+// pass 1 transforms the N1 columns (length N2, stride N1) and applies W_N^(n1*k2); pass 2
+// transforms the N2 rows (length N1) and writes them as columns (k = k2 + N2*k1).
+// Returns 1 (nothing done) when no such split exists.
+static int run_c2c_bailey(DevPlan &P, const void *in, long long istride, long long idist, void *out,
+                          long long ostride, long long odist, long long howmany, hipStream_t s)
+{
+    if (P.type != KFA_F32 || !env_flag("KISSFFT_AMD_FAST", true))
+        return 1;
+    const long long n = P.n;
+    const TileChoice *c1 = nullptr, *c2 = nullptr;
+    long long best1 = 0;
+    for (const TileVariant &v : tile_registry()) {
+        if (v.mode != TILE_ROWS_TO_COLS || v.inverse != P.inverse || n % v.n != 0)
+            continue;
+        const long long N1 = v.n, N2 = n / N1;
+        bool have2 = false;
+        for (const TileVariant &w : tile_registry())
+            have2 |= w.mode == TILE_COLS_TWID && w.inverse == P.inverse && w.n == N2;
+        if (!have2)
+            continue;
+        // most balanced split
+        auto score = [&](long long a) { return a > n / a ? a : n / a; };
+        if (best1 == 0 || score(N1) < score(best1))
+            best1 = N1;
+    }
+    if (best1 == 0)
+        return 1;
+    const long long N1 = best1, N2 = n / N1;
+    int rc;
+    if ((rc = tile_choice((int)N1, P.inverse, TILE_ROWS_TO_COLS, &c2)) ||
+        (rc = tile_choice((int)N2, P.inverse, TILE_COLS_TWID, &c1)))
+        return rc;
+    if (!c1->v || !c2->v)
+        return 1;
+    TileArgs a{};
+    if ((rc = fourstep_twiddles(n, P.inverse, &a.tw_hi, &a.tw_lo)))
+        return rc;
+    void *tmp = nullptr;
+    KFA_HIP(scratch_alloc(&tmp, 8 * (size_t)n * (size_t)howmany, s));
+    // pass 1: line (b, n1): element n2 at in[b*idist + (n1 + N1*n2)*istride] -> tmp
+    a.in = in;
+    a.out = tmp;
+    a.nlines = howmany * N1;
+    a.nlo = N1;
+    a.in_hi = idist;
+    a.in_lo = istride;
+    a.in_el = N1 * istride;
+    a.out_hi = n;
+    a.out_lo = 1;
+    a.out_el = N1;
+    rc = launch_tile(*c1, a, s);
+    if (rc == 0) {
+        // pass 2: line (b, k2): element n1 at tmp[b*n + n1 + N1*k2] -> out[k2 + N2*k1]
+        TileArgs b{};
+        b.in = tmp;
+        b.out = out;
+        b.nlines = howmany * N2;
+        b.nlo = N2;
+        b.in_hi = n;
+        b.in_lo = N1;
+        b.in_el = 1;
+        b.out_hi = odist;
+        b.out_lo = ostride;
+        b.out_el = N2 * ostride;
+        rc = launch_tile(*c2, b, s);
+    }
+    (void)hipFreeAsync(tmp, s);
+    return rc;
 }
 
 // Chooses (once per plan and kind) the fastest registered variant, or none.
@@ -322,16 +559,17 @@ int choose_fast(DevPlan &P, int kind, hipStream_t s, const FastChoice **out)
             lines = 64;
         const size_t bytes = cb * (size_t)(P.n + 1) * (size_t)lines * 2;
         void *bin = nullptr, *bout = nullptr, *brtw = nullptr;
-        KFA_HIP(hipMallocAsync(&bin, bytes, s));
-        KFA_HIP(hipMallocAsync(&bout, bytes, s));
-        KFA_HIP(hipMallocAsync(&brtw, cb * (size_t)(P.n / 2 + 1), s));
-        KFA_HIP(hipMemsetAsync(bin, 0, bytes, s));
-        KFA_HIP(hipMemsetAsync(brtw, 0, cb * (size_t)(P.n / 2 + 1), s));
+        KFA_HIP(scratch_alloc(&bin, bytes, s));
+        KFA_HIP(scratch_alloc(&bout, bytes, s));
+        KFA_HIP(scratch_alloc(&brtw, cb * (size_t)(P.n / 2 + 1), s));
+        // a non-zero constant pattern (float 0.0115, int16 15420): representative power draw
+        KFA_HIP(hipMemsetAsync(bin, 0x3c, bytes, s));
+        KFA_HIP(hipMemsetAsync(brtw, 0x3c, cb * (size_t)(P.n / 2 + 1), s));
         hipEvent_t e0, e1;
         KFA_HIP(hipEventCreate(&e0));
         KFA_HIP(hipEventCreate(&e1));
-        float best_ms = 0;
-        for (size_t i = 0; i < cand.size(); ++i) {
+        std::vector<float> t_best(cand.size(), 1e30f);
+        auto args_for = [&](size_t i) {
             FastArgs a{};
             a.in = bin;
             a.out = bout;
@@ -342,21 +580,32 @@ int choose_fast(DevPlan &P, int kind, hipStream_t s, const FastChoice **out)
             a.in_el = 1;
             a.out_hi = kind == FK_R2C ? P.n + 1 : P.n;
             a.out_el = 1;
-            cand[i]->launch(a, s);  // warm-up
-            KFA_HIP(hipEventRecord(e0, s));
-            for (int r = 0; r < 5; ++r)
-                cand[i]->launch(a, s);
-            KFA_HIP(hipEventRecord(e1, s));
-            KFA_HIP(hipEventSynchronize(e1));
-            float ms = 0;
-            KFA_HIP(hipEventElapsedTime(&ms, e0, e1));
-            // a later candidate must win by 3%: registration order breaks near-ties, so the
-            // choice does not flip with timing noise from run to run
-            if (i == 0 || ms < 0.97f * best_ms) {
-                best_ms = ms;
-                best = i;
+            return a;
+        };
+        for (size_t i = 0; i < cand.size(); ++i)
+            cand[i]->launch(args_for(i), s);  // warm-up (clocks, code objects)
+        // interleaved rounds, each candidate's best round counts: a clock or power
+        // transient during one candidate's turn does not decide the choice
+        for (int round = 0; round < 3; ++round) {
+            for (size_t i = 0; i < cand.size(); ++i) {
+                const FastArgs a = args_for(i);
+                KFA_HIP(hipEventRecord(e0, s));
+                for (int r = 0; r < 3; ++r)
+                    cand[i]->launch(a, s);
+                KFA_HIP(hipEventRecord(e1, s));
+                KFA_HIP(hipEventSynchronize(e1));
+                float ms = 0;
+                KFA_HIP(hipEventElapsedTime(&ms, e0, e1));
+                t_best[i] = std::min(t_best[i], ms);
             }
         }
+        // a later candidate must win by 2%: registration order breaks near-ties
+        for (size_t i = 1; i < cand.size(); ++i)
+            if (t_best[i] < 0.98f * t_best[best])
+                best = i;
+        if (env_flag("KISSFFT_AMD_VERBOSE", false))
+            for (size_t i = 0; i < cand.size(); ++i)
+                fprintf(stderr, "[kissfft_amd]   %-32s %8.1f us\n", cand[i]->name, 1e3f * t_best[i] / 3);
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         (void)hipFreeAsync(bin, s);
@@ -434,7 +683,7 @@ static int stage_in(const void *p, size_t bytes, hipStream_t s, Operand &op)
     }
     op.staged = true;
     op.host = p;
-    KFA_HIP(hipMallocAsync(&op.dev, bytes ? bytes : 1, s));
+    KFA_HIP(scratch_alloc(&op.dev, bytes ? bytes : 1, s));
     if (bytes) {
         KFA_HIP(hipMemcpyAsync(op.dev, p, bytes, hipMemcpyHostToDevice, s));
         g_h2d += (long long)bytes;
@@ -451,7 +700,7 @@ static int stage_out(void *p, size_t bytes, hipStream_t s, Operand &op)
     }
     op.staged = true;
     op.host = p;
-    KFA_HIP(hipMallocAsync(&op.dev, bytes ? bytes : 1, s));
+    KFA_HIP(scratch_alloc(&op.dev, bytes ? bytes : 1, s));
     return 0;
 }
 
@@ -548,6 +797,22 @@ int run_c2c(DevPlan &P, const void *in, long long istride, long long idist, void
     const int n = P.n;
     if (howmany <= 0 || n <= 0)
         return 0;
+    if (!P.single) {
+        // multi-pass lines use a temporary as large as their data: run large batches in
+        // chunks so the temporary stays within kScratchChunk bytes
+        const size_t cb = cpx_bytes(P.type);
+        const long long chunk = std::max(1LL, (long long)(kScratchChunk / (cb * (size_t)n)));
+        if (howmany > chunk) {
+            for (long long b = 0; b < howmany; b += chunk) {
+                const int rc = run_c2c(P, (const char *)in + (size_t)(b * idist) * cb, istride, idist,
+                                       (char *)out + (size_t)(b * odist) * cb, ostride, odist,
+                                       std::min(chunk, howmany - b), s);
+                if (rc)
+                    return rc;
+            }
+            return 0;
+        }
+    }
     if (P.single) {
         const FastChoice *fc = nullptr;
         int rc = choose_fast(P, FK_C2C, s, &fc);
@@ -582,10 +847,16 @@ int run_c2c(DevPlan &P, const void *in, long long istride, long long idist, void
         ll.colmode = 0;
         return launch_lines(P.type, ll, s);
     }
-    // four-step: pass 1 into a stream-ordered temporary, pass 2 into out
+    {
+        const int rc = run_c2c_bailey(P, in, istride, idist, out, ostride, odist, howmany, s);
+        if (rc <= 0)
+            return rc;
+    }
+    // four-step in the reference's stage order: pass 1 into a stream-ordered temporary,
+    // pass 2 into out
     void *tmp = nullptr;
     const size_t tbytes = cpx_bytes(P.type) * (size_t)n * (size_t)howmany;
-    KFA_HIP(hipMallocAsync(&tmp, tbytes, s));
+    KFA_HIP(scratch_alloc(&tmp, tbytes, s));
     LineLaunch a{};
     a.in = in;
     a.out = tmp;
@@ -647,8 +918,20 @@ static int run_real_multipass(DevPlan &H, const RealTables &R, int inverse, cons
                               void *out, long long out_hi, long long howmany, hipStream_t s)
 {
     const int nc = H.n;
+    const size_t cb = cpx_bytes(H.type);
+    const long long chunk = std::max(1LL, (long long)(kScratchChunk / (cb * (size_t)nc)));
+    if (howmany > chunk) {  // bounded temporaries, as for multi-pass complex lines
+        for (long long b = 0; b < howmany; b += chunk) {
+            const int rc = run_real_multipass(H, R, inverse, (const char *)in + (size_t)(b * in_hi) * cb, in_hi,
+                                              (char *)out + (size_t)(b * out_hi) * cb, out_hi,
+                                              std::min(chunk, howmany - b), s);
+            if (rc)
+                return rc;
+        }
+        return 0;
+    }
     void *tmp = nullptr;
-    KFA_HIP(hipMallocAsync(&tmp, cpx_bytes(H.type) * (size_t)nc * (size_t)howmany, s));
+    KFA_HIP(scratch_alloc(&tmp, cb * (size_t)nc * (size_t)howmany, s));
     auto pass = [&](int merge, const void *src, void *dst, long long hi) {
         switch (H.type) {
         case KFA_F32: return launch_real_pass<F32>(merge, src, dst, R.d_stw, nc, hi, howmany, s);
@@ -794,7 +1077,7 @@ int kfa_c2c(const kfa_cplan *pl, const void *in, long long istride, long long id
         } else {
             // device in == out with a layout the single pass cannot overwrite in place
             void *tmp = nullptr;
-            KFA_HIP(hipMallocAsync(&tmp, out_bytes, s));
+            KFA_HIP(scratch_alloc(&tmp, out_bytes, s));
             rc = run_c2c(*P, I.dev, istride, idist, tmp, 1, odist, howmany, s);
             if (rc == 0)
                 KFA_HIP(hipMemcpyAsync(O.dev, tmp, out_bytes, hipMemcpyDeviceToDevice, s));
@@ -870,11 +1153,29 @@ static int run_axis(DevPlan &P, const void *in, void *out, long long outer, long
                     hipStream_t s)
 {
     const long long len = P.n;
-    if (!P.single) {
-        // long axis: run the two-pass transform per outer block and inner column
-        if (inner != 1)
-            return fail(-2, "N-D axis of %lld points with inner stride %lld: not yet supported", len, inner);
+    if (inner == 1)  // the last axis: contiguous lines
         return run_c2c(P, in, 1, len, out, 1, len, outer, s);
+    if (!P.single)
+        return fail(-2, "N-D axis of %lld points with inner stride %lld: not yet supported", len, inner);
+    if (P.type == KFA_F32) {
+        const TileChoice *tc = nullptr;
+        int rc = tile_choice((int)len, P.inverse, TILE_COLS, &tc);
+        if (rc)
+            return rc;
+        if (tc->v) {
+            TileArgs a{};
+            a.in = in;
+            a.out = out;
+            a.nlines = outer * inner;
+            a.nlo = inner;
+            a.in_hi = len * inner;
+            a.in_lo = 1;
+            a.in_el = inner;
+            a.out_hi = len * inner;
+            a.out_lo = 1;
+            a.out_el = inner;
+            return launch_tile(*tc, a, s);
+        }
     }
     LineLaunch ll{};
     ll.in = in;
@@ -957,8 +1258,6 @@ int kfa_ndr(int ndims, const int *dims, const kfa_rplan *rpl, const kfa_cplan *a
             return rc;
         rows *= dims[k];
     }
-    if (!H->single)
-        return fail(-2, "real axis of %d points exceeds one LDS line (not yet supported)", dims[ndims - 1]);
     const long long nreal = dims[ndims - 1];
     const long long nbins = nreal / 2 + 1;
     const size_t sb = cpx_bytes(rpl->half.type) / 2;
@@ -983,7 +1282,7 @@ int kfa_ndr(int ndims, const int *dims, const kfa_rplan *rpl, const kfa_cplan *a
             // The input is left untouched: the axes run out of place into a temporary.
             void *tmp = nullptr;
             const size_t tb = cpx_bytes_;
-            KFA_HIP(hipMallocAsync(&tmp, tb, s));
+            KFA_HIP(scratch_alloc(&tmp, tb, s));
             long long outer = howmany, inner = rows * nbins;
             const void *src = I.dev;
             for (int k = 0; k + 1 < ndims && rc == 0; ++k) {
@@ -1031,8 +1330,26 @@ void kfa_cleanup(void)
             (void)hipFree(kv.second->d_stw);
             kv.second->d_stw = nullptr;
         }
+    for (auto &kv : C.cplans)
+        for (FastChoice &fc : kv.second->fast)
+            if (fc.d_stw) {
+                (void)hipFree(fc.d_stw);
+                fc.d_stw = nullptr;
+            }
     C.cplans.clear();
     C.rtabs.clear();
+    scratch_trim();
+    auto &T = tile_cache();
+    std::lock_guard<std::mutex> gt(T.mu);
+    for (auto &kv : T.choice)
+        if (kv.second.d_stw)
+            (void)hipFree(kv.second.d_stw);
+    for (auto &kv : T.fs) {
+        (void)hipFree(kv.second.first);
+        (void)hipFree(kv.second.second);
+    }
+    T.choice.clear();
+    T.fs.clear();
 }
 
 }  // extern "C"

@@ -29,3 +29,29 @@ struct FastVariant {
 const std::vector<FastVariant> &fast_registry();
 
 }  // namespace kfa
+
+namespace kfa {
+
+struct TileArgs;
+
+// Tile-kernel variants (kfa_tile.hpp): float, one length, direction and HBM mapping.
+enum { TILE_COLS = 0, TILE_COLS_TWID = 1, TILE_ROWS_TO_COLS = 2 };
+
+struct TileVariant {
+    int n;
+    int inverse;
+    int mode;     // TILE_COLS: column load / column store (N-D axes)
+                  // TILE_COLS_TWID: the same with the four-step twiddle (first pass)
+                  // TILE_ROWS_TO_COLS: row load / column store (four-step second pass)
+    int T, TL;    // threads per line, lines per workgroup
+    std::vector<int> radices;
+    int tw_pad;
+    int tw_len;
+    size_t lds;   // dynamic LDS bytes
+    const char *name;
+    int (*launch)(const TileArgs &, long long blocks, hipStream_t);
+};
+
+const std::vector<TileVariant> &tile_registry();
+
+}  // namespace kfa

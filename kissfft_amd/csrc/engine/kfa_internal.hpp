@@ -44,6 +44,10 @@ int fail(int code, const char *fmt, ...);
 size_t cpx_bytes(int type);
 int line_cap(int type);
 int current_device(int *dev);
+// stream-ordered scratch from the engine's per-device pool (free with hipFreeAsync)
+hipError_t scratch_alloc(void **p, size_t bytes, hipStream_t s);
+// multi-pass transforms process batches in chunks whose temporary fits this many bytes
+constexpr size_t kScratchChunk = (size_t)1 << 30;
 bool device_accessible(const void *p);
 
 // device plans (cached; thread-safe)

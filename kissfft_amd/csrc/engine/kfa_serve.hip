@@ -17,6 +17,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <condition_variable>
 #include <deque>
 #include <map>
@@ -62,6 +64,7 @@ struct Server;
 struct LaunchRec {
     Server *srv;
     std::vector<std::shared_ptr<ServeReq>> reqs;
+    double t_enq0 = 0, t_enq1 = 0;  // host time around enqueueing the group
 };
 
 struct Server {
@@ -79,6 +82,7 @@ struct Server {
     std::unordered_map<long long, std::shared_ptr<ServeReq>> live;
     long long next_id = 1;
     bool stopping = false;
+    bool trace = false;  // KISSFFT_AMD_VERBOSE=2: one line per completed launch group
     int inflight_launches = 0;
     std::thread th;
 
@@ -117,6 +121,14 @@ static void on_launch_done(void *p)
     }
     S->cv_done.notify_all();
     S->cv_work.notify_all();
+    if (S->trace) {
+        const ServeReq &r0 = *rec->reqs[0];
+        long long lines = 0;
+        for (auto &r : rec->reqs)
+            lines += r->howmany;
+        fprintf(stderr, "[kissfft_amd serve] kind=%d n=%d reqs=%zu lines=%lld enqueue=%.3f ms done=+%.3f ms\n",
+                r0.kind, r0.n, rec->reqs.size(), lines, 1e3 * (rec->t_enq1 - rec->t_enq0), 1e3 * (t - rec->t_enq1));
+    }
     delete rec;
 }
 
@@ -234,19 +246,20 @@ void Server::run()
                 std::vector<std::shared_ptr<ServeReq>> grp(all.begin() + off,
                                                            all.begin() + std::min(all.size(), off + (size_t)max_group));
                 hipStream_t s = streams[next_stream++ % streams.size()];
+                const double t_enq0 = now_s();
                 int rc = 0;
                 for (auto &r : grp) {
                     r->d_in = const_cast<void *>(r->in);
                     r->d_out = r->out;
                     if (r->stage_in && rc == 0) {
-                        rc = hipMallocAsync(&r->d_in, r->in_bytes ? r->in_bytes : 1, s) == hipSuccess ? 0 : -3;
+                        rc = scratch_alloc(&r->d_in, r->in_bytes ? r->in_bytes : 1, s) == hipSuccess ? 0 : -3;
                         if (rc == 0 && r->in_bytes)
                             rc = hipMemcpyAsync(r->d_in, r->in, r->in_bytes, hipMemcpyHostToDevice, s) == hipSuccess
                                      ? 0
                                      : -3;
                     }
                     if (r->stage_out && rc == 0)
-                        rc = hipMallocAsync(&r->d_out, r->out_bytes ? r->out_bytes : 1, s) == hipSuccess ? 0 : -3;
+                        rc = scratch_alloc(&r->d_out, r->out_bytes ? r->out_bytes : 1, s) == hipSuccess ? 0 : -3;
                 }
                 if (rc == 0)
                     rc = launch_group(grp, s);
@@ -258,7 +271,7 @@ void Server::run()
                     if (r->stage_out && r->d_out)
                         (void)hipFreeAsync(r->d_out, s);
                 }
-                LaunchRec *rec = new LaunchRec{this, grp};
+                LaunchRec *rec = new LaunchRec{this, grp, t_enq0, now_s()};
                 {
                     std::lock_guard<std::mutex> g(mu);
                     // a failed group still completes behind the stream (its staging copies may
@@ -307,6 +320,10 @@ void *kfa_server_create(int type, long long budget_bytes, int nstreams, int max_
         budget_bytes = (long long)(fr / 10 * 8);  // 80% of the HBM free now
     }
     S->budget = (size_t)budget_bytes;
+    {
+        const char *v = getenv("KISSFFT_AMD_VERBOSE");
+        S->trace = v && atoi(v) >= 2;
+    }
     S->max_group = (max_group <= 0 || max_group > KFA_MAX_SEG) ? KFA_MAX_SEG : max_group;
     if (nstreams <= 0)
         nstreams = 4;
@@ -362,9 +379,13 @@ long long kfa_server_submit(void *srv, int kind, const kfa_cplan *cp, const kfa_
     r->stage_out = !device_accessible(out);
     r->need = (r->stage_in ? r->in_bytes : 0) + (r->stage_out ? r->out_bytes : 0);
     // multi-pass scratch: the four-step temporary (complex: n values per transform; real:
-    // n/2, plus the n/2 of the split/merge temporary)
-    if (!r->P->single)
-        r->need += cpx_bytes(S->type) * (size_t)nfft * (size_t)howmany;
+    // n/2, plus the n/2 of the split/merge temporary), each bounded by the chunking of
+    // multi-pass batches
+    if (!r->P->single) {
+        const bool real = kind == KFA_SERVE_R2C || kind == KFA_SERVE_C2R;
+        const size_t one = cpx_bytes(S->type) * (size_t)r->P->n * (size_t)howmany;
+        r->need += (real ? 2 : 1) * std::min(one, kScratchChunk);
+    }
     r->t_submit = now_s();
     {
         std::lock_guard<std::mutex> g(S->mu);

@@ -1,0 +1,139 @@
+// kfa_tile.hpp -- the tile kernel: compile-time radix schedules on strided lines.
+//
+// A workgroup owns a tile of TL lines of N points.  Line l = (hi, lo), lo = l % nlo, reads
+// element e at in[hi*in_hi + lo*in_lo + e*in_el] and writes it at
+// out[hi*out_hi + lo*out_lo + e*out_el].  The tile moves between HBM and LDS in two
+// phases of its own, each with the thread mapping that makes wave accesses contiguous:
+//   * "column" phases (LCOL / SCOL): consecutive lanes take consecutive lines at the same
+//     element -- the layout of the non-last axes of an N-D array (kiss_fftnd.c:172-177
+//     walks the same columns) and of the first pass of the four-step split;
+//   * "row" phases: consecutive lanes take consecutive elements of one line.
+// In between, the stages of the radix schedule RS run LDS -> registers -> LDS exactly as
+// in the fast line kernel (FastStage, T threads per line).  Lines sit N+1 values apart in
+// LDS so that lanes working on neighbouring lines fall in different banks.
+//
+// TWID multiplies element e of line l by W^(lo*e), W = exp(-+2*pi*i/tw_n), on the way out:
+// the twiddle step between the two passes of the four-step split (Bailey, "FFTs in
+// external or hierarchical memory", 1990):  with n = n1 + N1*n2 and k = k2 + N2*k1,
+//   X[k2 + N2*k1] = sum_n1 W_N1^(n1*k1) * W_N^(n1*k2) * sum_n2 x[n1 + N1*n2] W_N2^(n2*k2).
+// W^a is read as hi[a >> 10] * lo[a & 1023] from two small tables (computed in FP64,
+// rounded once).  The float path only: fixed-point long lines keep the reference's own
+// stage order (the general line kernel's two-pass split).
+#pragma once
+
+#include "kfa_fast.hpp"
+
+namespace kfa {
+
+struct TileArgs {
+    const void *in;
+    void *out;
+    const void *stw;     // stage table of the schedule (the fast kernels' layout)
+    const void *tw_hi;   // TWID: W^(q*1024), q <= tw_n/1024
+    const void *tw_lo;   // TWID: W^r, r < 1024
+    long long nlines, nlo;
+    long long in_hi, in_lo, in_el;
+    long long out_hi, out_lo, out_el;
+    int xcd_group;       // > 1: tiles g*G .. g*G+G-1 (which share cache lines) run on one XCD
+};
+
+template <class A, bool INV, int T, class RS, int S>
+__device__ __forceinline__ void tile_stage(typename A::store2 *lb, const typename A::store2 *stw, int t)
+{
+    typedef FastStage<A, INV, T, RS, S, false> F;
+    typename F::C v[F::J][F::p];
+    F::template gather<false>(v, lb, 1, t, true);
+    __syncthreads();  // every input read before the buffer is overwritten
+    F::compute(v, stw, t);
+    F::template store<false>(v, lb, 1, t, true);
+    __syncthreads();
+}
+
+template <class A, bool INV, int T, class RS, int... S>
+__device__ __forceinline__ void tile_stages(std::integer_sequence<int, S...>, typename A::store2 *lb,
+                                            const typename A::store2 *stw, int t)
+{
+    (tile_stage<A, INV, T, RS, S>(lb, stw, t), ...);
+}
+
+// Workgroup -> tile.  Workgroups are dealt to the 8 XCDs round-robin; with G tiles sharing
+// each cache line, the G tiles of a group go to one XCD back to back, so the line is
+// fetched into that XCD's L2 once.
+__device__ __forceinline__ long long tile_of_block(long long b, long long ntiles, int G)
+{
+    if (G <= 1 || ntiles % (8LL * G) != 0)
+        return b;
+    const long long x = b % 8, s = b / 8;
+    return ((s / G) * 8 + x) * G + s % G;
+}
+
+template <class A, bool INV, int T, int TL, bool LCOL, bool SCOL, bool TWID, class RS>
+__global__ void __launch_bounds__(T *TL) k_tile(TileArgs a)
+{
+    typedef typename A::store2 St;
+    typedef Ops<A> O;
+    typedef Cx<A> C;
+    constexpr int N = RS::N();
+    constexpr int NS = N + 1;
+    constexpr int NT = T * TL;
+    extern __shared__ __align__(16) unsigned char kfa_tile_smem[];
+    St *buf = reinterpret_cast<St *>(kfa_tile_smem);
+    __shared__ long long s_ib[TL], s_ob[TL], s_lo[TL];
+
+    const int tid = threadIdx.x;
+    const long long ntiles = (a.nlines + TL - 1) / TL;
+    const long long l0 = tile_of_block(blockIdx.x, ntiles, a.xcd_group) * TL;
+    const int nl = (int)((a.nlines - l0) < TL ? (a.nlines - l0) : TL);
+    if (tid < TL) {
+        const long long gl = l0 + (tid < nl ? tid : 0);
+        const long long hi = gl / a.nlo, lo = gl - hi * a.nlo;
+        s_ib[tid] = hi * a.in_hi + lo * a.in_lo;
+        s_ob[tid] = hi * a.out_hi + lo * a.out_lo;
+        s_lo[tid] = lo;
+    }
+    __syncthreads();
+
+    const St *in = reinterpret_cast<const St *>(a.in);
+    St *out = reinterpret_cast<St *>(a.out);
+    for (int w = tid; w < TL * N; w += NT) {
+        int line, e;
+        if (LCOL) {
+            line = w % TL;
+            e = w / TL;
+        } else {
+            line = w / N;
+            e = w - line * N;
+        }
+        if (line < nl)
+            buf[line * NS + e] = in[s_ib[line] + (long long)e * a.in_el];
+    }
+    __syncthreads();
+
+    const int line = tid / T, t = tid - line * T;
+    tile_stages<A, INV, T, RS>(std::make_integer_sequence<int, RS::K>{}, buf + line * NS,
+                               reinterpret_cast<const St *>(a.stw), t);
+
+    const St *whi = reinterpret_cast<const St *>(a.tw_hi);
+    const St *wlo = reinterpret_cast<const St *>(a.tw_lo);
+    for (int w = tid; w < TL * N; w += NT) {
+        int ln, e;
+        if (SCOL) {
+            ln = w % TL;
+            e = w / TL;
+        } else {
+            ln = w / N;
+            e = w - ln * N;
+        }
+        if (ln >= nl)
+            continue;
+        if constexpr (TWID) {
+            const unsigned long long ex = (unsigned long long)s_lo[ln] * (unsigned)e;  // < tw_n
+            const C wv = O::cmul(O::load(whi[ex >> 10]), O::load(wlo[ex & 1023]));
+            out[s_ob[ln] + (long long)e * a.out_el] = O::pack(O::cmul(O::load(buf[ln * NS + e]), wv));
+        } else {
+            out[s_ob[ln] + (long long)e * a.out_el] = buf[ln * NS + e];
+        }
+    }
+}
+
+}  // namespace kfa

@@ -1,0 +1,81 @@
+// kfa_tile_reg.hip -- the instantiated tile-kernel variants (float) and their launchers.
+//
+// Lengths cover the non-last axes of the N-D workloads (64, 96, 1024, ...) and the halves
+// of the four-step split of long lines (128 .. 1024, N = N1 * N2 up to 2^20 and beyond).
+// Each length is registered in three HBM mappings (column / column + twiddle / row-in
+// column-out) and both directions.
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+
+#include "kfa_fast_reg.hpp"
+#include "kfa_tile.hpp"
+
+namespace kfa {
+
+template <bool INV, int T, int TL, bool LCOL, bool SCOL, bool TWID, class RS>
+static int launch_tile(const TileArgs &a, long long blocks, hipStream_t s)
+{
+    const size_t lds = (size_t)TL * (RS::N() + 1) * sizeof(float2);
+    static std::once_flag once;
+    std::call_once(once, [=] {
+        (void)hipFuncSetAttribute((const void *)k_tile<F32, INV, T, TL, LCOL, SCOL, TWID, RS>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    });
+    if (blocks <= 0)
+        return 0;
+    hipLaunchKernelGGL((k_tile<F32, INV, T, TL, LCOL, SCOL, TWID, RS>), dim3((unsigned)blocks), dim3(T * TL), lds,
+                       s, a);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+template <int T, int TL, int... R>
+static void add_tile(std::vector<TileVariant> &v, const char *name)
+{
+    constexpr int G = 2;  // 16-byte stage-table rows of float2
+    typedef Radices<G, R...> RS;
+    static_assert(T * TL <= 1024, "workgroup too large");
+    static_assert((size_t)TL * (RS::N() + 1) * 8 <= 160 * 1024, "tile exceeds LDS");
+    for (int mode = 0; mode < 3; ++mode)
+        for (int inv = 0; inv < 2; ++inv) {
+            TileVariant f;
+            f.n = RS::N();
+            f.inverse = inv;
+            f.mode = mode;
+            f.T = T;
+            f.TL = TL;
+            f.radices = {R...};
+            f.tw_pad = G;
+            f.tw_len = RS::tw_len();
+            f.lds = (size_t)TL * (RS::N() + 1) * 8;
+            f.name = name;
+            if (mode == TILE_COLS)
+                f.launch = inv ? &launch_tile<true, T, TL, true, true, false, RS>
+                               : &launch_tile<false, T, TL, true, true, false, RS>;
+            else if (mode == TILE_COLS_TWID)
+                f.launch = inv ? &launch_tile<true, T, TL, true, true, true, RS>
+                               : &launch_tile<false, T, TL, true, true, true, RS>;
+            else
+                f.launch = inv ? &launch_tile<true, T, TL, false, true, false, RS>
+                               : &launch_tile<false, T, TL, false, true, false, RS>;
+            v.push_back(f);
+        }
+}
+
+const std::vector<TileVariant> &tile_registry()
+{
+    static std::vector<TileVariant> *reg = [] {
+        auto *v = new std::vector<TileVariant>();
+        add_tile<8, 64, 8, 8>(*v, "tile64_88_t8x64");
+        add_tile<16, 32, 8, 4, 3>(*v, "tile96_843_t16x32");
+        add_tile<16, 32, 8, 16>(*v, "tile128_816_t16x32");
+        add_tile<16, 16, 16, 16>(*v, "tile256_1616_t16x16");
+        add_tile<64, 16, 8, 8, 8>(*v, "tile512_888_t64x16");
+        add_tile<64, 16, 16, 16, 4>(*v, "tile1024_16164_t64x16");
+        add_tile<64, 8, 16, 16, 4>(*v, "tile1024_16164_t64x8");
+        return v;
+    }();
+    return *reg;
+}
+
+}  // namespace kfa

@@ -83,3 +83,40 @@ def algorithmic_bytes(kind: str, n_or_dims, scalar_bytes: int = 4) -> int:
     tot = int(np.prod(dims))
     bins = tot // dims[-1] * (dims[-1] // 2 + 1)
     return scalar_bytes * tot + 2 * scalar_bytes * bins
+
+
+# Serving stream: Poisson arrivals of mixed requests.
+SERVE_SIZES = (256, 1000, 1009, 3072, 65536, 1048576)
+SERVE_KINDS = ("c2c", "r2c", "inverse")
+SERVE_MAX_BATCH = 4096
+
+
+def serving_stream(count: int, rate_per_s: float, seed: int = 6) -> list[dict]:
+    """`count` requests: arrival times (Poisson process of `rate_per_s`), kind, n, batch.
+
+    kind is uniform over c2c / r2c / inverse; "inverse" is the real inverse (c2r), except at
+    odd n (1009), where neither real kind exists (real transforms need even n) and r2c maps
+    to the complex forward, inverse to the complex inverse.  n is uniform over SERVE_SIZES;
+    batch = floor(exp(U(0, ln 4096))), log-uniform in [1, 4096].  Uniforms come from
+    splitmix64 (four per request)."""
+    z = splitmix64(seed, 4 * count)
+    u = ((z >> np.uint64(11)).astype(np.float64) * 2.0 ** -53).reshape(count, 4)
+    out, t = [], 0.0
+    for i in range(count):
+        t += -np.log1p(-u[i, 0]) / rate_per_s
+        kind = SERVE_KINDS[min(2, int(u[i, 1] * 3))]
+        n = SERVE_SIZES[min(5, int(u[i, 2] * 6))]
+        if n % 2:
+            kind = {"c2c": "c2c", "r2c": "c2c", "inverse": "c2c_inv"}[kind]
+        elif kind == "inverse":
+            kind = "c2r"
+        batch = max(1, min(SERVE_MAX_BATCH, int(np.exp(u[i, 3] * np.log(SERVE_MAX_BATCH)))))
+        out.append(dict(t=t, kind=kind, n=n, batch=batch))
+    return out
+
+
+def request_bytes(kind: str, n: int, batch: int, scalar_bytes: int = 4) -> tuple[int, int]:
+    """(input bytes, output bytes) of one dense request."""
+    cpx, real, half = 2 * scalar_bytes * n, scalar_bytes * n, 2 * scalar_bytes * (n // 2 + 1)
+    i, o = {"c2c": (cpx, cpx), "c2c_inv": (cpx, cpx), "r2c": (real, half), "c2r": (half, real)}[kind]
+    return i * batch, o * batch

@@ -133,14 +133,32 @@ def test_float_c2c_prime_1009(lf, golden_float):
     assert err <= float_bound(golden_float, "c2c_1009")
 
 
-@pytest.mark.parametrize("n,howmany", [(16384, 3), (65536, 4), (262144, 2), (1048576, 2)])
-def test_float_c2c_long_lines(lf, n, howmany):
+@pytest.mark.parametrize("n,howmany", [(16384, 3), (32768, 3), (65536, 4), (131072, 2), (262144, 2),
+                                       (1048576, 2), (107520, 2)])
+@pytest.mark.parametrize("inverse", [False, True])
+def test_float_c2c_long_lines(lf, n, howmany, inverse):
+    # four-step split: N1*N2 tile passes where both halves are registered, else the
+    # general two-pass split in the reference's stage order (107520 = 3*5*7*1024)
     x = W.complex_pm1(6, n, howmany)
-    say("case float long", n)
-    got = cplx(run_batch(lf, n, False, x))
-    err = rel_l2(got, np.fft.fft(cplx(x), axis=-1))
+    say("case float long", n, "inverse", inverse)
+    got = cplx(run_batch(lf, n, inverse, x))
+    want = np.fft.ifft(cplx(x), axis=-1) * n if inverse else np.fft.fft(cplx(x), axis=-1)
+    err = rel_l2(got, want)
     say("rel-L2", err)
     assert err <= ABS_FLOAT_BOUND
+
+
+def test_float_c2c_long_lines_strided(lf):
+    # istride / idist / odist through both four-step passes
+    n, howmany, s = 65536, 3, 2
+    rng = np.random.default_rng(17)
+    idist, odist = n * s + 5, n + 3
+    buf = rng.uniform(-1, 1, (howmany * idist, 2)).astype(np.float32)
+    out = np.zeros((howmany * odist, 2), np.float32)
+    lf.fft_batch(lf.fft_alloc(n), buf, out, howmany, idist, odist, in_stride=s)
+    for b in range(howmany):
+        xb = cplx(buf[b * idist: b * idist + n * s: s])
+        assert rel_l2(cplx(out[b * odist: b * odist + n]), np.fft.fft(xb)) <= ABS_FLOAT_BOUND, b
 
 
 def test_float_stride_and_in_place(lf):
@@ -205,3 +223,44 @@ def test_kfc_cached_calls(lf):
     assert rel_l2(cplx(z), 512 * cplx(x)) <= ABS_FLOAT_BOUND
     lf.L.kfc_cleanup()
     assert lf.L.kfc_cached_count() == 0
+
+
+def test_float_long_lines_chunked_batches(lf):
+    # batches whose four-step temporary exceeds the 1 GiB scratch bound run in chunks
+    # (2^20 complex: 128 transforms per chunk; 2^20 real: 256): sampled transforms against
+    # FP64 across chunk edges, every transform through the round trip
+    import torch
+    dev = torch.device("cuda:0")
+    n, B = 1 << 20, 130
+    g = torch.Generator(device=dev).manual_seed(77)
+    x = torch.rand(B, n, 2, device=dev, generator=g) * 2 - 1
+    y = torch.empty_like(x)
+    lf.fft_batch(lf.fft_alloc(n), x, y, B, n, n)
+    torch.cuda.synchronize()
+    for b in (0, 127, 128, 129):
+        say("chunked c2c sample", b)
+        assert rel_l2(cplx(y[b].cpu().numpy()), np.fft.fft(cplx(x[b].cpu().numpy()))) <= ABS_FLOAT_BOUND, b
+    z = torch.empty_like(x)
+    lf.fft_batch(lf.fft_alloc(n, True), y, z, B, n, n)
+    torch.cuda.synchronize()
+    rt = float(torch.linalg.vector_norm((z.double() - n * x.double()).flatten()) /
+               torch.linalg.vector_norm((n * x.double()).flatten()))
+    say("chunked c2c round trip", rt)
+    assert rt <= 2 * ABS_FLOAT_BOUND
+    del x, y, z
+    B = 260
+    xr = torch.rand(B, n, device=dev, generator=g) * 2 - 1
+    X = torch.empty(B, n // 2 + 1, 2, device=dev)
+    lf.fftr_batch(lf.fftr_alloc(n), xr, X, B, n, n // 2 + 1)
+    torch.cuda.synchronize()
+    for b in (0, 255, 256, 259):
+        say("chunked r2c sample", b)
+        want = np.fft.rfft(xr[b].cpu().numpy().astype(np.float64))
+        assert rel_l2(cplx(X[b].cpu().numpy()), want) <= ABS_FLOAT_BOUND, b
+    yr = torch.empty_like(xr)
+    lf.fftri_batch(lf.fftr_alloc(n, True), X, yr, B, n // 2 + 1, n)
+    torch.cuda.synchronize()
+    rt = float(torch.linalg.vector_norm((yr.double() - n * xr.double()).flatten()) /
+               torch.linalg.vector_norm((n * xr.double()).flatten()))
+    say("chunked real round trip", rt)
+    assert rt <= 2 * ABS_FLOAT_BOUND

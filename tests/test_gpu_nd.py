@@ -125,6 +125,31 @@ def test_float_nd_shapes():
         assert rel_l2(y, np.prod(dims) * x.astype(np.float64)) <= 2 * ABS_FLOAT_BOUND, dims
 
 
+@pytest.mark.parametrize("dims", [(96, 64), (1024, 24), (256, 10), (512, 3), (128, 5, 7), (64, 64, 64),
+                                  (3, 1024, 9)])
+def test_float_nd_tile_axes(dims):
+    # non-last axes of registered lengths run as column tiles
+    lib = kissfft_amd.load("float")
+    x = W.complex_pm1(44, int(np.prod(dims)), 3).reshape((3,) + dims + (2,))
+    axes = tuple(range(1, len(dims) + 1))
+    for inv in (False, True):
+        say("case nd tiles", dims, inv)
+        got = cplx(nd(lib, dims, x, inv))
+        want = np.fft.ifftn(cplx(x), axes=axes) * np.prod(dims) if inv else np.fft.fftn(cplx(x), axes=axes)
+        assert rel_l2(got, want) <= ABS_FLOAT_BOUND, (dims, inv)
+
+
+def test_float_ndr_long_real_axis():
+    # real last axis longer than one LDS line (multi-pass real rows) under a column axis
+    lib = kissfft_amd.load("float")
+    dims = (64, 65536)
+    x = W.normal(45, int(np.prod(dims)) * 2).reshape((2,) + dims)
+    X = ndr(lib, dims, x)
+    assert rel_l2(cplx(X), np.fft.rfftn(x.astype(np.float64), axes=(1, 2))) <= ABS_FLOAT_BOUND
+    y = ndri(lib, dims, X)
+    assert rel_l2(y, np.prod(dims) * x.astype(np.float64)) <= 2 * ABS_FLOAT_BOUND
+
+
 def test_nd_in_place_single_call():
     lib = kissfft_amd.load("float")
     dims = (8, 12, 10)

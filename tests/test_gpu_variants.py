@@ -106,3 +106,52 @@ def test_forced_variant(typ, kind, n, name, monkeypatch):
                 assert rel_l2(y, want) <= 4e-7
     finally:
         lib.L.kiss_fft_cleanup()
+
+
+TREG = os.path.join(ROOT, "kissfft_amd", "csrc", "engine", "kfa_tile_reg.hip")
+
+
+def tile_registry():
+    out = []
+    for m in re.finditer(r"add_tile<([^>]*)>\(\*v,\s*\"(\w+)\"\)", open(TREG).read()):
+        a = [int(x) for x in m.group(1).split(",")]
+        out.append((int(np.prod(a[2:])), m.group(2)))
+    return out
+
+
+TILES = tile_registry()
+
+
+def test_tile_registry_parsed():
+    assert len(TILES) >= 6
+
+
+@pytest.mark.parametrize("n,name", TILES, ids=[t[1] for t in TILES])
+def test_forced_tile(n, name, monkeypatch):
+    """Each tile variant forced: as the column pass of a 2-D transform (n x 40), and as both
+    four-step passes of an n*n line where that line is longer than one LDS line."""
+    lib = kissfft_amd.load("float")
+    monkeypatch.setenv("KISSFFT_AMD_TILE", name)
+    monkeypatch.setenv("KISSFFT_AMD_VERBOSE", "1")
+    lib.L.kiss_fft_cleanup()
+    try:
+        dims = (n, 40)
+        x = W.complex_pm1(70, n * 40, 2).reshape((2,) + dims + (2,))
+        for inv in (False, True):
+            say("tile", name, "nd", dims, inv)
+            cfg = lib.fftnd_alloc(dims, inv)
+            y = np.zeros_like(x)
+            lib.fftnd_batch(cfg, x, y, 2)
+            want = np.fft.ifftn(cplx(x), axes=(1, 2)) * n * 40 if inv else np.fft.fftn(cplx(x), axes=(1, 2))
+            assert rel_l2(cplx(y), want) <= ABS_FLOAT_BOUND, (name, inv)
+        N = n * n
+        if N > 8192:
+            xs = W.complex_pm1(71, N, 2)
+            for inv in (False, True):
+                say("tile", name, "four-step", N, inv)
+                ys = np.zeros_like(xs)
+                lib.fft_batch(lib.fft_alloc(N, inv), xs, ys, 2, N, N)
+                want = np.fft.ifft(cplx(xs), axis=-1) * N if inv else np.fft.fft(cplx(xs), axis=-1)
+                assert rel_l2(cplx(ys), want) <= ABS_FLOAT_BOUND, (name, inv)
+    finally:
+        lib.L.kiss_fft_cleanup()
