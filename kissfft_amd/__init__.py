@@ -21,7 +21,7 @@ import numpy as np
 
 from . import build as _build
 
-__all__ = ["Library", "Server", "load", "SCALARS", "KissFFTError"]
+__all__ = ["Library", "Server", "MultiBatch", "load", "SCALARS", "KissFFTError"]
 
 SCALARS = {"float": np.float32, "int16_t": np.int16, "int32_t": np.int32}
 
@@ -145,6 +145,12 @@ class Library:
             "kiss_fft_server_get_stats": ([vp, ctypes.POINTER(ServerStats)], ci),
             "kiss_fft_server_reset_stats": ([vp], None),
             "kiss_fft_server_destroy": ([vp], None),
+            "kiss_fft_multi_alloc": ([ctypes.POINTER(MultiItem), ci], vp),
+            "kiss_fft_multi_exec": ([vp, vp], ci),
+            "kiss_fft_multi_capture": ([vp, vp], ci),
+            "kiss_fft_multi_replay": ([vp, vp], ci),
+            "kiss_fft_multi_info": ([vp, ctypes.POINTER(ll), ctypes.POINTER(ci), ctypes.POINTER(ci)], ci),
+            "kiss_fft_multi_free": ([vp], None),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -271,6 +277,59 @@ class Library:
 
     def server(self, hbm_budget_bytes: int = 0, streams: int = 0, max_group: int = 0) -> "Server":
         return Server(self, hbm_budget_bytes, streams, max_group)
+
+    def multi(self, items) -> "MultiBatch":
+        """items: (kind, nfft, fin, fout, howmany) tuples, device buffers."""
+        return MultiBatch(self, items)
+
+
+class MultiItem(ctypes.Structure):
+    """kiss_fft_multi_item (include/kiss_fft_multi.h)."""
+    _fields_ = [("kind", ctypes.c_int), ("nfft", ctypes.c_int), ("fin", ctypes.c_void_p),
+                ("fout", ctypes.c_void_p), ("howmany", ctypes.c_longlong)]
+
+
+class MultiBatch:
+    """A kiss_fft_multi: a fixed set of batches of mixed lengths and kinds, one launch.
+
+    Kinds as for the server (C2C_FWD, C2C_INV, R2C, C2R).  The buffers must stay alive as
+    long as the multi-batch is used."""
+    C2C_FWD, C2C_INV, R2C, C2R = 0, 1, 2, 3
+
+    def __init__(self, lib: Library, items):
+        self.lib = lib
+        arr = (MultiItem * len(items))()
+        for i, (kind, nfft, fin, fout, howmany) in enumerate(items):
+            arr[i] = MultiItem(kind, nfft, _addr(fin), _addr(fout), howmany)
+        self._keep = list(items)
+        self.ptr = lib.L.kiss_fft_multi_alloc(arr, len(items))
+        if not self.ptr:
+            raise KissFFTError(f"kiss_fft_multi_alloc failed: {lib.last_error()}")
+
+    def exec(self, stream=None) -> None:
+        self.lib.check(self.lib.L.kiss_fft_multi_exec(self.ptr, _stream_ptr(stream)), "kiss_fft_multi_exec")
+
+    def capture(self, stream) -> None:
+        self.lib.check(self.lib.L.kiss_fft_multi_capture(self.ptr, _stream_ptr(stream)), "kiss_fft_multi_capture")
+
+    def replay(self, stream=None) -> None:
+        self.lib.check(self.lib.L.kiss_fft_multi_replay(self.ptr, _stream_ptr(stream)), "kiss_fft_multi_replay")
+
+    def info(self) -> dict:
+        t, g, p = ctypes.c_longlong(), ctypes.c_int(), ctypes.c_int()
+        self.lib.L.kiss_fft_multi_info(self.ptr, ctypes.byref(t), ctypes.byref(g), ctypes.byref(p))
+        return {"tiles": t.value, "grid": g.value, "persistent_items": p.value}
+
+    def close(self) -> None:
+        if getattr(self, "ptr", None):
+            self.lib.L.kiss_fft_multi_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class ServerStats(ctypes.Structure):

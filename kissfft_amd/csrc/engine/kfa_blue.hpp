@@ -1,0 +1,82 @@
+// kfa_blue.hpp -- Bluestein's chirp-z transform for float lengths with large prime factors.
+//
+// The reference transforms a prime factor p by the direct O(p^2) butterfly
+// (kf_bfly_generic, kiss_fft.c:191-233).  For float lines whose schedule contains a large
+// prime this kernel computes the same DFT through a cyclic convolution of power-of-two
+// length M >= 2N-1 (Bluestein, "A linear filtering approach to the computation of the
+// discrete Fourier transform", 1970):
+//   w[n] = exp(-+ i*pi*n^2/N),   X[k] = w[k] * sum_n (x[n] w[n]) conj(w[k-n])
+// i.e. X = w * IFFT_M( FFT_M(x*w, zero-padded) * B ),  B = FFT_M(conj(w) wrapped) / M.
+// One line per T threads, entirely in LDS: the chirp multiply on load, the forward M-point
+// schedule, the pointwise product with B, the inverse schedule, the chirp on store.
+// w (N entries) and B (M entries) are computed in FP64 on the host and rounded once.
+#pragma once
+
+#include "kfa_tile.hpp"
+
+namespace kfa {
+
+struct BlueArgs {
+    const void *in;
+    void *out;
+    const void *stw_f, *stw_i;  // stage tables of the M-point schedule, forward and inverse
+    const void *chirp;          // w[n], n < N
+    const void *bfft;           // B[m], m < M
+    int n;
+    long long nlines;
+    long long in_hi, in_el, out_hi, out_el;
+};
+
+template <int T, int LPB, class RS>
+__global__ void __launch_bounds__(T *LPB) k_blue(BlueArgs a)
+{
+    typedef F32::store2 St;
+    typedef Ops<F32> O;
+    typedef Cx<F32> C;
+    constexpr int M = RS::N();
+    constexpr int IT = M / T;
+    static_assert(M % T == 0, "threads per line must divide M");
+    extern __shared__ __align__(16) unsigned char kfa_blue_smem[];
+    const int line = threadIdx.x / T, t = threadIdx.x - line * T;
+    St *buf = reinterpret_cast<St *>(kfa_blue_smem) + line * (M + 1);
+    const long long gl = (long long)blockIdx.x * LPB + line;
+    const bool ok = gl < a.nlines;
+    const int n = a.n;
+    const St *chirp = reinterpret_cast<const St *>(a.chirp);
+    const St *bf = reinterpret_cast<const St *>(a.bfft);
+
+    {
+        const St *src = reinterpret_cast<const St *>(a.in) + (ok ? gl : 0) * a.in_hi;
+        C v[IT];
+#pragma unroll
+        for (int k = 0; k < IT; ++k) {
+            const int i = t + k * T;
+            v[k] = (ok && i < n) ? O::cmul(O::load(src[(long long)i * a.in_el]), O::load(chirp[i])) : O::zero();
+        }
+#pragma unroll
+        for (int k = 0; k < IT; ++k)
+            buf[t + k * T] = O::pack(v[k]);
+    }
+    __syncthreads();
+    tile_stages<F32, false, T, RS>(std::make_integer_sequence<int, RS::K>{}, buf,
+                                   reinterpret_cast<const St *>(a.stw_f), t);
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+        const int i = t + k * T;
+        buf[i] = O::pack(O::cmul(O::load(buf[i]), O::load(bf[i])));
+    }
+    __syncthreads();
+    tile_stages<F32, true, T, RS>(std::make_integer_sequence<int, RS::K>{}, buf,
+                                  reinterpret_cast<const St *>(a.stw_i), t);
+    if (!ok)
+        return;
+    St *dst = reinterpret_cast<St *>(a.out) + gl * a.out_hi;
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+        const int i = t + k * T;
+        if (i < n)
+            dst[(long long)i * a.out_el] = O::pack(O::cmul(O::load(buf[i]), O::load(chirp[i])));
+    }
+}
+
+}  // namespace kfa

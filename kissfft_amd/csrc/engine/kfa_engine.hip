@@ -32,6 +32,7 @@
 #include "kfa_fast_reg.hpp"
 #include "kfa_internal.hpp"
 #include "kfa_lines.hpp"
+#include "kfa_blue.hpp"
 #include "kfa_tile.hpp"
 
 namespace kfa {
@@ -440,6 +441,135 @@ static int launch_tile(const TileChoice &tc, TileArgs a, hipStream_t s)
     return tc.v->launch(a, tiles, s) ? fail(-3, "tile kernel launch failed") : 0;
 }
 
+// ------------------------------------------------------------------------------------
+// Bluestein (float lengths with a large prime factor)
+// ------------------------------------------------------------------------------------
+struct BlueChoice {
+    const BlueVariant *v = nullptr;
+    void *stw_f = nullptr, *stw_i = nullptr, *chirp = nullptr, *bfft = nullptr;
+};
+
+static std::mutex g_blue_mu;
+static std::map<std::tuple<int, int, int>, BlueChoice> g_blue;  // dev, n, inverse
+
+// In-place radix-2 FFT in FP64 (forward: exp(-2*pi*i*j*k/M)), M a power of two.
+static void fft_pow2_f64(std::vector<double> &re, std::vector<double> &im)
+{
+    const size_t M = re.size();
+    for (size_t i = 1, j = 0; i < M; ++i) {
+        size_t bit = M >> 1;
+        for (; j & bit; bit >>= 1)
+            j ^= bit;
+        j ^= bit;
+        if (i < j) {
+            std::swap(re[i], re[j]);
+            std::swap(im[i], im[j]);
+        }
+    }
+    const double pi = 3.14159265358979323846264338327;
+    for (size_t len = 2; len <= M; len <<= 1) {
+        for (size_t k = 0; k < len / 2; ++k) {
+            const double ang = -2 * pi * (double)k / (double)len;
+            const double wr = cos(ang), wi = sin(ang);
+            for (size_t i = 0; i < M; i += len) {
+                const size_t a = i + k, b = i + k + len / 2;
+                const double xr = re[b] * wr - im[b] * wi, xi = re[b] * wi + im[b] * wr;
+                re[b] = re[a] - xr;
+                im[b] = im[a] - xi;
+                re[a] += xr;
+                im[a] += xi;
+            }
+        }
+    }
+}
+
+// The Bluestein variant and tables for a float plan, or v == nullptr when the plan has no
+// prime factor above 16 or no registered convolution length fits.
+static int blue_choice(const DevPlan &P, const BlueChoice **out)
+{
+    static const BlueChoice none;
+    *out = &none;
+    if (P.type != KFA_F32 || !env_flag("KISSFFT_AMD_BLUESTEIN", true))
+        return 0;
+    int pmax = 0;
+    for (int p : P.radix)
+        pmax = std::max(pmax, p);
+    if (pmax <= 16)
+        return 0;
+    int dev;
+    int rc = current_device(&dev);
+    if (rc)
+        return rc;
+    std::lock_guard<std::mutex> g(g_blue_mu);
+    const auto key = std::make_tuple(dev, P.n, P.inverse);
+    auto it = g_blue.find(key);
+    if (it != g_blue.end()) {
+        *out = &it->second;
+        return 0;
+    }
+    BlueChoice bc;
+    const long long n = P.n;
+    for (const BlueVariant &v : blue_registry())
+        if (v.M >= 2 * n - 1 && (!bc.v || v.M < bc.v->M))
+            bc.v = &v;
+    if (bc.v) {
+        const int M = bc.v->M;
+        const double pi = 3.14159265358979323846264338327;
+        const double sg = P.inverse ? 1.0 : -1.0;
+        std::vector<double> wr(n), wi(n);
+        std::vector<float> hw(2 * n);
+        for (long long i = 0; i < n; ++i) {
+            const long long q = (i * i) % (2 * n);  // n^2 mod 2N: exact phase reduction
+            const double ph = sg * pi * (double)q / (double)n;
+            wr[i] = cos(ph);
+            wi[i] = sin(ph);
+            hw[2 * i] = (float)wr[i];
+            hw[2 * i + 1] = (float)wi[i];
+        }
+        std::vector<double> br(M, 0.0), bi(M, 0.0);
+        for (long long m = 0; m < n; ++m) {
+            br[m] = wr[m];
+            bi[m] = -wi[m];
+            if (m) {
+                br[M - m] = wr[m];
+                bi[M - m] = -wi[m];
+            }
+        }
+        fft_pow2_f64(br, bi);
+        std::vector<float> hb(2 * M);
+        for (int m = 0; m < M; ++m) {
+            hb[2 * m] = (float)(br[m] / M);
+            hb[2 * m + 1] = (float)(bi[m] / M);
+        }
+        const std::vector<unsigned char> tf = float_twiddles(M, 0, M, 1), ti = float_twiddles(M, 1, M, 1);
+        if ((rc = make_stage_table_raw(tf.data(), KFA_F32, M, bc.v->radices, bc.v->tw_pad, 0, &bc.stw_f)) ||
+            (rc = make_stage_table_raw(ti.data(), KFA_F32, M, bc.v->radices, bc.v->tw_pad, 0, &bc.stw_i)))
+            return rc;
+        KFA_HIP(hipMalloc(&bc.chirp, hw.size() * 4));
+        KFA_HIP(hipMalloc(&bc.bfft, hb.size() * 4));
+        KFA_HIP(hipMemcpy(bc.chirp, hw.data(), hw.size() * 4, hipMemcpyHostToDevice));
+        KFA_HIP(hipMemcpy(bc.bfft, hb.data(), hb.size() * 4, hipMemcpyHostToDevice));
+        if (env_flag("KISSFFT_AMD_VERBOSE", false))
+            fprintf(stderr, "[kissfft_amd] n=%lld inverse=%d: Bluestein %s\n", n, P.inverse, bc.v->name);
+    }
+    auto &slot = g_blue[key];
+    slot = bc;
+    *out = &slot;
+    return 0;
+}
+
+static void blue_release()
+{
+    std::lock_guard<std::mutex> g(g_blue_mu);
+    for (auto &kv : g_blue) {
+        BlueChoice &b = kv.second;
+        for (void *p : {b.stw_f, b.stw_i, b.chirp, b.bfft})
+            if (p)
+                (void)hipFree(p);
+    }
+    g_blue.clear();
+}
+
 // Float four-step (Bailey) for n = N1*N2 with both halves registered as tile variants:
 // pass 1 transforms the N1 columns (length N2, stride N1) and applies W_N^(n1*k2); pass 2
 // transforms the N2 rows (length N1) and writes them as columns (k = k2 + N2*k1).
@@ -831,6 +961,26 @@ int run_c2c(DevPlan &P, const void *in, long long istride, long long idist, void
             ++g_launches;
             return fc->v->launch(a, s) ? fail(-3, "fast kernel launch failed") : 0;
         }
+        const BlueChoice *bc = nullptr;
+        if ((rc = blue_choice(P, &bc)))
+            return rc;
+        if (bc->v) {
+            BlueArgs b{};
+            b.in = in;
+            b.out = out;
+            b.stw_f = bc->stw_f;
+            b.stw_i = bc->stw_i;
+            b.chirp = bc->chirp;
+            b.bfft = bc->bfft;
+            b.n = n;
+            b.nlines = howmany;
+            b.in_hi = idist;
+            b.in_el = istride;
+            b.out_hi = odist;
+            b.out_el = ostride;
+            ++g_launches;
+            return bc->v->launch(b, s) ? fail(-3, "Bluestein kernel launch failed") : 0;
+        }
         LineLaunch ll{};
         ll.in = in;
         ll.out = out;
@@ -1020,6 +1170,9 @@ static int end_call(hipStream_t s, int sync, int rc)
     }
     return rc;
 }
+
+int call_begin(void *stream, hipStream_t *s, int *dev) { return begin(s, stream, dev); }
+int call_end(hipStream_t s, int sync, int rc) { return end_call(s, sync, rc); }
 
 }  // namespace kfa
 
@@ -1350,6 +1503,7 @@ void kfa_cleanup(void)
     }
     T.choice.clear();
     T.fs.clear();
+    blue_release();
 }
 
 }  // extern "C"

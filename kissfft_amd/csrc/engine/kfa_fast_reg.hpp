@@ -55,3 +55,22 @@ struct TileVariant {
 const std::vector<TileVariant> &tile_registry();
 
 }  // namespace kfa
+
+namespace kfa {
+
+struct BlueArgs;
+
+// Bluestein variants (kfa_blue.hpp): one convolution length M, float.
+struct BlueVariant {
+    int M;
+    int T, LPB;
+    std::vector<int> radices;
+    int tw_pad;
+    size_t lds;
+    const char *name;
+    int (*launch)(const BlueArgs &, hipStream_t);
+};
+
+const std::vector<BlueVariant> &blue_registry();
+
+}  // namespace kfa

@@ -62,6 +62,11 @@ int run_c2c(DevPlan &P, const void *in, long long istride, long long idist, void
 int run_real_rows(DevPlan &H, const RealTables &R, int inverse, const void *in, long long in_hi, void *out,
                   long long out_hi, long long howmany, hipStream_t s);
 
+// entry / exit of a C-ABI call: the stream to use (the caller's, or the calling thread's
+// own blocking stream for NULL) and the optional synchronisation at the end
+int call_begin(void *stream, hipStream_t *s, int *dev);
+int call_end(hipStream_t s, int sync, int rc);
+
 extern std::atomic<long long> g_plans_built, g_plan_hits, g_launches, g_h2d, g_d2h;
 
 }  // namespace kfa

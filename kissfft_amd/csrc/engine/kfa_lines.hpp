@@ -300,21 +300,18 @@ __global__ void __launch_bounds__(256) k_real_merge(const typename A::store2 *__
     }
 }
 
-// The line kernel.  Dynamic LDS: 2 * TL * L complex values.
+// One tile of TL lines starting at line l0: HBM -> LDS, every stage, LDS -> HBM.
+// bufA holds 2 * TL * L complex values.
 template <class A>
-__global__ void __launch_bounds__(256) k_lines(const typename A::store2 *__restrict__ in,
-                                               typename A::store2 *__restrict__ out,
-                                               const typename A::store2 *__restrict__ tw,
-                                               const typename A::store2 *__restrict__ stw,
-                                               LinePlan pl, LineIO io, int TL, int colmode, int prologue,
-                                               int epilogue)
+__device__ __forceinline__ void lines_tile(const typename A::store2 *__restrict__ in,
+                                           typename A::store2 *__restrict__ out,
+                                           const typename A::store2 *__restrict__ tw,
+                                           const typename A::store2 *__restrict__ stw, const LinePlan &pl,
+                                           const LineIO &io, int TL, int colmode, int prologue, int epilogue,
+                                           long long l0, typename A::store2 *bufA)
 {
     typedef typename A::store2 S;
-    extern __shared__ __align__(16) unsigned char kfa_smem[];
-    S *bufA = reinterpret_cast<S *>(kfa_smem);
     S *bufB = bufA + (size_t)TL * pl.L;
-
-    const long long l0 = (long long)blockIdx.x * TL;
     LineCtx<A> cx;
     cx.TL = TL;
     cx.L = pl.L;
@@ -359,6 +356,57 @@ __global__ void __launch_bounds__(256) k_lines(const typename A::store2 *__restr
             const long long base = (l / io.nlo) * io.out_hi + (l % io.nlo) * io.out_lo;
             out[base + (long long)t * io.out_el] = src[cx.idx(line, t)];
         }
+    }
+}
+
+// The line kernel: one tile per workgroup.  Dynamic LDS: 2 * TL * L complex values.
+template <class A>
+__global__ void __launch_bounds__(256) k_lines(const typename A::store2 *__restrict__ in,
+                                               typename A::store2 *__restrict__ out,
+                                               const typename A::store2 *__restrict__ tw,
+                                               const typename A::store2 *__restrict__ stw,
+                                               LinePlan pl, LineIO io, int TL, int colmode, int prologue,
+                                               int epilogue)
+{
+    extern __shared__ __align__(16) unsigned char kfa_smem[];
+    lines_tile<A>(in, out, tw, stw, pl, io, TL, colmode, prologue, epilogue, (long long)blockIdx.x * TL,
+                  reinterpret_cast<typename A::store2 *>(kfa_smem));
+}
+
+// Several independent batches of different lengths and kinds in one persistent launch:
+// item i owns tiles [tile0, tile0 + ntiles) of TL lines each; every workgroup walks the
+// global tile list with stride gridDim.x (no counters, so the launch can be captured and
+// replayed as is).
+struct MultiItem {
+    LinePlan pl;
+    const void *tw, *stw;
+    const void *in;
+    void *out;
+    LineIO io;
+    long long tile0;
+    int TL, prologue, epilogue;
+};
+
+template <class A>
+__global__ void __launch_bounds__(256) k_multi(const MultiItem *__restrict__ items, int nitems, long long ntiles)
+{
+    typedef typename A::store2 S;
+    extern __shared__ __align__(16) unsigned char kfa_msmem[];
+    S *buf = reinterpret_cast<S *>(kfa_msmem);
+    for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        int lo = 0, hi = nitems - 1;  // last item with tile0 <= tile
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (items[mid].tile0 <= tile)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const MultiItem &it = items[lo];
+        lines_tile<A>(reinterpret_cast<const S *>(it.in), reinterpret_cast<S *>(it.out),
+                      reinterpret_cast<const S *>(it.tw), reinterpret_cast<const S *>(it.stw), it.pl, it.io, it.TL,
+                      0, it.prologue, it.epilogue, (tile - it.tile0) * it.TL, buf);
+        __syncthreads();  // the next tile reuses the buffers
     }
 }
 

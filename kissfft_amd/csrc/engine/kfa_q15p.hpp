@@ -55,9 +55,15 @@ __device__ __forceinline__ u32 lohi(u32 x, u32 y) { return (x & 0xffffu) | (y & 
 // bits 15..30 of two sums, packed (sround of each component, truncated to int16)
 __device__ __forceinline__ u32 round_pack(int re, int im) { return lohi((u32)(re >> 15), (u32)im << 1); }
 
+// a.lo*b.lo + a.hi*b.hi + 2^14 (signed 16-bit halves, 32-bit wrapping sum).  Written as
+// the three-address VOP3P form with the rounding constant in a scalar register: the
+// builtin selects the two-address accumulate form, which needs a register copy of the
+// constant before every product.
 __device__ __forceinline__ int dot(u32 a, u32 b)
 {
-    return __builtin_amdgcn_sdot2(as_v(a), as_v(b), 1 << 14, false);
+    int r;
+    asm("v_dot2_i32_i16 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(1 << 14));
+    return r;
 }
 
 // complex product with a pre-arranged twiddle pair (w.r, -w.i), (w.i, w.r)

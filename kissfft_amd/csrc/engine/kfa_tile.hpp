@@ -75,64 +75,76 @@ __global__ void __launch_bounds__(T *TL) k_tile(TileArgs a)
     typedef Cx<A> C;
     constexpr int N = RS::N();
     constexpr int NS = N + 1;
-    constexpr int NT = T * TL;
+    constexpr int IT = N / T;  // elements per thread in the HBM phases
+    static_assert(N % T == 0, "threads per line must divide the length");
     extern __shared__ __align__(16) unsigned char kfa_tile_smem[];
     St *buf = reinterpret_cast<St *>(kfa_tile_smem);
-    __shared__ long long s_ib[TL], s_ob[TL], s_lo[TL];
 
     const int tid = threadIdx.x;
     const long long ntiles = (a.nlines + TL - 1) / TL;
     const long long l0 = tile_of_block(blockIdx.x, ntiles, a.xcd_group) * TL;
     const int nl = (int)((a.nlines - l0) < TL ? (a.nlines - l0) : TL);
-    if (tid < TL) {
-        const long long gl = l0 + (tid < nl ? tid : 0);
+
+    // HBM phases: each thread owns one line and IT of its elements.  Column mapping:
+    // lanes take consecutive lines (line = tid % TL), element e0 + k*T; row mapping:
+    // T lanes take consecutive elements of one line (line = tid / T).
+    auto line_of = [&](bool col) { return col ? tid % TL : tid / T; };
+    auto elem0 = [&](bool col) { return col ? tid / TL : tid % T; };
+
+    {
+        const int ln = line_of(LCOL), e0 = elem0(LCOL);
+        const bool ok = ln < nl;
+        const long long gl = l0 + (ok ? ln : 0);
         const long long hi = gl / a.nlo, lo = gl - hi * a.nlo;
-        s_ib[tid] = hi * a.in_hi + lo * a.in_lo;
-        s_ob[tid] = hi * a.out_hi + lo * a.out_lo;
-        s_lo[tid] = lo;
+        const St *src = reinterpret_cast<const St *>(a.in) + hi * a.in_hi + lo * a.in_lo + e0 * a.in_el;
+        const long long step = (long long)T * a.in_el;
+        St v[IT];
+#pragma unroll
+        for (int k = 0; k < IT; ++k)
+            if (ok)
+                v[k] = src[k * step];
+#pragma unroll
+        for (int k = 0; k < IT; ++k)
+            if (ok)
+                buf[ln * NS + e0 + k * T] = v[k];
     }
     __syncthreads();
 
-    const St *in = reinterpret_cast<const St *>(a.in);
-    St *out = reinterpret_cast<St *>(a.out);
-    for (int w = tid; w < TL * N; w += NT) {
-        int line, e;
-        if (LCOL) {
-            line = w % TL;
-            e = w / TL;
-        } else {
-            line = w / N;
-            e = w - line * N;
-        }
-        if (line < nl)
-            buf[line * NS + e] = in[s_ib[line] + (long long)e * a.in_el];
+    {
+        const int line = tid / T, t = tid - line * T;
+        tile_stages<A, INV, T, RS>(std::make_integer_sequence<int, RS::K>{}, buf + line * NS,
+                                   reinterpret_cast<const St *>(a.stw), t);
     }
-    __syncthreads();
 
-    const int line = tid / T, t = tid - line * T;
-    tile_stages<A, INV, T, RS>(std::make_integer_sequence<int, RS::K>{}, buf + line * NS,
-                               reinterpret_cast<const St *>(a.stw), t);
-
-    const St *whi = reinterpret_cast<const St *>(a.tw_hi);
-    const St *wlo = reinterpret_cast<const St *>(a.tw_lo);
-    for (int w = tid; w < TL * N; w += NT) {
-        int ln, e;
-        if (SCOL) {
-            ln = w % TL;
-            e = w / TL;
-        } else {
-            ln = w / N;
-            e = w - ln * N;
-        }
+    {
+        const int ln = line_of(SCOL), e0 = elem0(SCOL);
         if (ln >= nl)
-            continue;
+            return;
+        const long long gl = l0 + ln;
+        const long long hi = gl / a.nlo, lo = gl - hi * a.nlo;
+        St *dst = reinterpret_cast<St *>(a.out) + hi * a.out_hi + lo * a.out_lo + e0 * a.out_el;
+        const long long step = (long long)T * a.out_el;
+        C v[IT];
+#pragma unroll
+        for (int k = 0; k < IT; ++k)
+            v[k] = O::load(buf[ln * NS + e0 + k * T]);
         if constexpr (TWID) {
-            const unsigned long long ex = (unsigned long long)s_lo[ln] * (unsigned)e;  // < tw_n
-            const C wv = O::cmul(O::load(whi[ex >> 10]), O::load(wlo[ex & 1023]));
-            out[s_ob[ln] + (long long)e * a.out_el] = O::pack(O::cmul(O::load(buf[ln * NS + e]), wv));
-        } else {
-            out[s_ob[ln] + (long long)e * a.out_el] = buf[ln * NS + e];
+            const St *whi = reinterpret_cast<const St *>(a.tw_hi);
+            const St *wlo = reinterpret_cast<const St *>(a.tw_lo);
+            C wh[IT], wl[IT];
+#pragma unroll
+            for (int k = 0; k < IT; ++k) {
+                const unsigned long long ex = (unsigned long long)lo * (unsigned)(e0 + k * T);  // < tw_n
+                wh[k] = O::load(whi[ex >> 10]);
+                wl[k] = O::load(wlo[ex & 1023]);
+            }
+#pragma unroll
+            for (int k = 0; k < IT; ++k)
+                v[k] = O::cmul(v[k], O::cmul(wh[k], wl[k]));
         }
+#pragma unroll
+        for (int k = 0; k < IT; ++k)
+            dst[k * step] = O::pack(v[k]);
     }
 }
 

@@ -1,4 +1,5 @@
-// kfa_tile_reg.hip -- the instantiated tile-kernel variants (float) and their launchers.
+// kfa_tile_reg.hip -- the instantiated tile-kernel and Bluestein variants (float) and
+// their launchers.
 //
 // Lengths cover the non-last axes of the N-D workloads (64, 96, 1024, ...) and the halves
 // of the four-step split of long lines (128 .. 1024, N = N1 * N2 up to 2^20 and beyond).
@@ -9,6 +10,7 @@
 #include <mutex>
 
 #include "kfa_fast_reg.hpp"
+#include "kfa_blue.hpp"
 #include "kfa_tile.hpp"
 
 namespace kfa {
@@ -73,6 +75,55 @@ const std::vector<TileVariant> &tile_registry()
         add_tile<64, 16, 8, 8, 8>(*v, "tile512_888_t64x16");
         add_tile<64, 16, 16, 16, 4>(*v, "tile1024_16164_t64x16");
         add_tile<64, 8, 16, 16, 4>(*v, "tile1024_16164_t64x8");
+        return v;
+    }();
+    return *reg;
+}
+
+template <int T, int LPB, class RS>
+static int launch_blue(const BlueArgs &a, hipStream_t s)
+{
+    const size_t lds = (size_t)LPB * (RS::N() + 1) * sizeof(float2);
+    static std::once_flag once;
+    std::call_once(once, [=] {
+        (void)hipFuncSetAttribute((const void *)k_blue<T, LPB, RS>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds);
+    });
+    const long long blocks = (a.nlines + LPB - 1) / LPB;
+    if (blocks <= 0)
+        return 0;
+    hipLaunchKernelGGL((k_blue<T, LPB, RS>), dim3((unsigned)blocks), dim3(T * LPB), lds, s, a);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+template <int T, int LPB, int... R>
+static void add_blue(std::vector<BlueVariant> &v, const char *name)
+{
+    typedef Radices<2, R...> RS;
+    static_assert(T * LPB <= 1024, "workgroup too large");
+    BlueVariant f;
+    f.M = RS::N();
+    f.T = T;
+    f.LPB = LPB;
+    f.radices = {R...};
+    f.tw_pad = 2;
+    f.lds = (size_t)LPB * (RS::N() + 1) * 8;
+    f.name = name;
+    f.launch = &launch_blue<T, LPB, RS>;
+    v.push_back(f);
+}
+
+const std::vector<BlueVariant> &blue_registry()
+{
+    static std::vector<BlueVariant> *reg = [] {
+        auto *v = new std::vector<BlueVariant>();
+        add_blue<8, 8, 8, 8>(*v, "blue64_88_t8x8");
+        add_blue<16, 4, 8, 16>(*v, "blue128_816_t16x4");
+        add_blue<16, 4, 16, 16>(*v, "blue256_1616_t16x4");
+        add_blue<64, 2, 8, 8, 8>(*v, "blue512_888_t64x2");
+        add_blue<64, 2, 16, 16, 4>(*v, "blue1024_16164_t64x2");
+        add_blue<128, 2, 16, 16, 8>(*v, "blue2048_16168_t128x2");
+        add_blue<256, 1, 16, 16, 16>(*v, "blue4096_161616_t256");
         return v;
     }();
     return *reg;

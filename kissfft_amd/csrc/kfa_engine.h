@@ -100,6 +100,24 @@ typedef struct {
 } kfa_stats;
 KFA_HIDDEN void kfa_get_stats(kfa_stats *s);
 
+/* Multi-batch: several batches of different lengths / kinds in one persistent launch. */
+typedef struct {
+    int kind;              /* KFA_SERVE_* */
+    int nfft;
+    const kfa_cplan *cp;   /* complex kinds */
+    const kfa_rplan *rp;   /* real kinds */
+    const void *in;        /* device pointers, howmany dense transforms */
+    void *out;
+    long long howmany;
+} kfa_multi_item;
+
+KFA_HIDDEN void *kfa_multi_create(int type, const kfa_multi_item *items, int nitems);
+KFA_HIDDEN int kfa_multi_exec(void *m, void *stream, int sync);
+KFA_HIDDEN int kfa_multi_capture(void *m, void *stream);
+KFA_HIDDEN int kfa_multi_replay(void *m, void *stream, int sync);
+KFA_HIDDEN int kfa_multi_info(void *m, long long *tiles, int *grid, int *persistent_items);
+KFA_HIDDEN void kfa_multi_destroy(void *m);
+
 #ifdef __cplusplus
 }
 #endif

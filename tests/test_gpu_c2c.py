@@ -264,3 +264,30 @@ def test_float_long_lines_chunked_batches(lf):
                torch.linalg.vector_norm((n * xr.double()).flatten()))
     say("chunked real round trip", rt)
     assert rt <= 2 * ABS_FLOAT_BOUND
+
+
+@pytest.mark.parametrize("n", [17, 34, 1009, 1011, 2018, 2039, 2053])
+def test_float_large_prime_lengths(lf, n, monkeypatch):
+    # lengths with a prime factor above 16: Bluestein convolution (M = 64 .. 4096) for
+    # n <= 2048, the direct generic butterfly above; both directions, strided input, and
+    # the generic path forced off Bluestein for comparison
+    x = W.complex_pm1(90 + n, n, 8)
+    for inv in (False, True):
+        say("case prime", n, inv)
+        got = cplx(run_batch(lf, n, inv, x))
+        want = np.fft.ifft(cplx(x), axis=-1) * n if inv else np.fft.fft(cplx(x), axis=-1)
+        err = rel_l2(got, want)
+        say("rel-L2", err)
+        assert err <= ABS_FLOAT_BOUND, (n, inv)
+    s = 3
+    buf = W.complex_pm1(91, n * s, 1)[0]
+    out = np.zeros((n, 2), np.float32)
+    lf.kiss_fft_stride(lf.fft_alloc(n), buf, out, s)
+    assert rel_l2(cplx(out), np.fft.fft(cplx(buf[::s]))) <= ABS_FLOAT_BOUND
+    monkeypatch.setenv("KISSFFT_AMD_BLUESTEIN", "0")
+    lf.L.kiss_fft_cleanup()
+    try:
+        got = cplx(run_batch(lf, n, False, x))
+        assert rel_l2(got, np.fft.fft(cplx(x), axis=-1)) <= ABS_FLOAT_BOUND
+    finally:
+        lf.L.kiss_fft_cleanup()

@@ -1,0 +1,128 @@
+"""GPU tests of the multi-batch (include/kiss_fft_multi.h): mixed lengths and kinds in one
+persistent launch, and its hipGraph capture / replay."""
+import sys
+
+import numpy as np
+import pytest
+
+import kissfft_amd
+from helpers import ABS_FLOAT_BOUND, cplx, rel_l2
+from kissfft_amd import workloads as W
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+C2C, C2C_INV, R2C, C2R = 0, 1, 2, 3
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def make(torch, dev, kind, n, b, seed, dtype):
+    """Input / output tensors of one item (complex as trailing pairs)."""
+    if dtype == "float":
+        shape_in = {C2C: (b, n, 2), C2C_INV: (b, n, 2), R2C: (b, n), C2R: (b, n // 2 + 1, 2)}[kind]
+        x = torch.from_numpy(W.uniform_pm1(seed, int(np.prod(shape_in))).reshape(shape_in)).to(dev)
+        tdt = torch.float32
+    else:
+        shape_in = {C2C: (b, n, 2), C2C_INV: (b, n, 2), R2C: (b, n), C2R: (b, n // 2 + 1, 2)}[kind]
+        x = torch.from_numpy(W.q15_uniform(seed, int(np.prod(shape_in))).reshape(shape_in)).to(dev)
+        tdt = torch.int16
+    shape_out = {C2C: (b, n, 2), C2C_INV: (b, n, 2), R2C: (b, n // 2 + 1, 2), C2R: (b, n)}[kind]
+    return x, torch.empty(shape_out, dtype=tdt, device=dev)
+
+
+def fp64_ref(kind, n, x):
+    if kind == C2C:
+        return np.fft.fft(cplx(x), axis=-1)
+    if kind == C2C_INV:
+        return np.fft.ifft(cplx(x), axis=-1) * n
+    if kind == R2C:
+        return np.fft.rfft(x.astype(np.float64), axis=-1)
+    return np.fft.irfft(cplx(x), n=n, axis=-1) * n
+
+
+FLOAT_ITEMS = [(C2C, 256, 100), (C2C, 1009, 7), (C2C_INV, 1200, 33), (R2C, 4096, 10), (C2R, 4000, 9),
+               (C2C, 60, 500), (C2C, 1001, 3), (R2C, 6, 4), (C2C, 1, 3), (C2R, 2, 5), (C2C, 8192, 2),
+               (C2C, 65536, 2), (R2C, 40000, 2)]
+
+
+def test_float_mix_one_launch():
+    import torch
+    lib = kissfft_amd.load("float")
+    dev = torch.device("cuda:0")
+    bufs = [make(torch, dev, k, n, b, 200 + i, "float") for i, (k, n, b) in enumerate(FLOAT_ITEMS)]
+    m = lib.multi([(k, n, x, y, b) for (k, n, b), (x, y) in zip(FLOAT_ITEMS, bufs)])
+    info = m.info()
+    say("multi info", info)
+    assert info["persistent_items"] == len(FLOAT_ITEMS) - 2  # 65536 and 40000 are multi-pass
+    assert info["grid"] > 0 and info["tiles"] > 0
+    m.exec()
+    for (k, n, b), (x, y) in zip(FLOAT_ITEMS, bufs):
+        say("item", k, n, b)
+        got = y.cpu().numpy()
+        got = got if k == C2R else cplx(got)
+        if k == C2C and n == 1:
+            np.testing.assert_array_equal(y.cpu().numpy(), x.cpu().numpy())
+            continue
+        assert rel_l2(got, fp64_ref(k, n, x.cpu().numpy())) <= ABS_FLOAT_BOUND, (k, n)
+    # a long item cannot be captured (it allocates scratch)
+    with pytest.raises(kissfft_amd.KissFFTError):
+        m.capture(torch.cuda.Stream())
+    m.close()
+
+
+Q15_ITEMS = [(C2C, 512, 64), (C2C, 1200, 5), (R2C, 4096, 3), (C2R, 500, 4), (C2C, 1001, 2), (C2C_INV, 60, 9),
+             (C2C, 12, 17), (C2C, 65536, 1)]
+
+
+def test_q15_mix_bit_exact():
+    import torch
+    lib = kissfft_amd.load("int16_t")
+    O = Oracle("int16_t")
+    dev = torch.device("cuda:0")
+    bufs = [make(torch, dev, k, n, b, 300 + i, "int16_t") for i, (k, n, b) in enumerate(Q15_ITEMS)]
+    m = lib.multi([(k, n, x, y, b) for (k, n, b), (x, y) in zip(Q15_ITEMS, bufs)])
+    m.exec()
+    for (k, n, b), (x, y) in zip(Q15_ITEMS, bufs):
+        say("q15 item", k, n, b)
+        xs = x.cpu().numpy()
+        want = {C2C: lambda: O.fft(xs), C2C_INV: lambda: O.fft(xs, inverse=True), R2C: lambda: O.fftr(xs),
+                C2R: lambda: O.fftri(xs, n)}[k]()
+        np.testing.assert_array_equal(y.cpu().numpy(), want, err_msg=f"kind {k} n {n}")
+    m.close()
+
+
+def test_graph_capture_and_replays():
+    # replays interleaved with eager work on the same stream; every replay is checked
+    # against an eager execution of the same multi-batch on the same inputs
+    import torch
+    lib = kissfft_amd.load("float")
+    dev = torch.device("cuda:0")
+    items = [(C2C, 256, 40), (C2C, 1009, 3), (R2C, 4096, 6), (C2R, 2000, 5), (C2C_INV, 3072, 4), (C2C, 60, 90)]
+    bufs = [make(torch, dev, k, n, b, 400 + i, "float") for i, (k, n, b) in enumerate(items)]
+    m = lib.multi([(k, n, x, y, b) for (k, n, b), (x, y) in zip(items, bufs)])
+    st = torch.cuda.Stream()
+    m.capture(st)
+    for rep in range(3):
+        with torch.cuda.stream(st):
+            for x, _ in bufs:
+                x.mul_(1.5).add_(0.25)  # eager work between replays
+            for _, y in bufs:
+                y.fill_(7.0)
+        m.replay(st)
+        st.synchronize()
+        replayed = [y.clone() for _, y in bufs]
+        for _, y in bufs:
+            y.fill_(-3.0)
+        torch.cuda.synchronize()
+        m.exec()  # eager, the calling thread's stream, synchronised
+        for (k, n, b), (x, y), r in zip(items, bufs, replayed):
+            say("replay", rep, "item", k, n)
+            assert torch.equal(r, y), (rep, k, n)
+            got = y.cpu().numpy()
+            got = got if k == C2R else cplx(got)
+            assert rel_l2(got, fp64_ref(k, n, x.cpu().numpy())) <= ABS_FLOAT_BOUND, (rep, k, n)
+    m.close()
