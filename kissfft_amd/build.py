@@ -10,6 +10,8 @@ Layout (kissfft_amd/lib/):
                                      libkissfft-<type>.so.131, as the reference's
                                      Makefile:89-113 names its per-datatype libraries
     libkissfft-<type>.so             symlink to the above
+    fftutil-<type>                   the stream tool (tools/fftutil.c protocol), linked
+                                     against the library; alias fastfilt-<type>
 
 The host layer is compiled with gcc and without -ffast-math (the fixed-point twiddle
 tables depend on exact FP64 rounding).  All HIP code is compiled with packed-FP32
@@ -100,7 +102,21 @@ def build(datatypes=None, verbose: bool = True) -> list[str]:
             os.remove(link)
         os.symlink(soname, link)
         built.append(real)
+        built.append(build_fftutil(dt, verbose))
     return built
+
+
+def build_fftutil(dt: str, verbose: bool = True) -> str:
+    """fftutil-<type>: the reference's stream tool (tools/fftutil.c protocol) over the
+    library, plus the name the reference's test harness runs it under (fastfilt-<type>)."""
+    exe = os.path.join(LIBDIR, f"fftutil-{dt}")
+    _run([CC, "-O2", "-std=gnu99", "-Wall", *DATATYPES[dt], "-I", INCLUDE, os.path.join(CSRC, "tools", "fftutil.c"),
+          "-L", LIBDIR, f"-lkissfft-{dt}", "-Wl,-rpath,$ORIGIN", "-o", exe], verbose)
+    alias = os.path.join(LIBDIR, f"fastfilt-{dt}")
+    if os.path.lexists(alias):
+        os.remove(alias)
+    os.symlink(os.path.basename(exe), alias)
+    return exe
 
 
 def build_oracle(verbose: bool = True) -> None:

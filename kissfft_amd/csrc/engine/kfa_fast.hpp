@@ -79,6 +79,40 @@ struct FastArgs {
     FastSeg seg[KFA_MAX_SEG];    // ascending `first`, seg[0].first == 0 (passed by value)
 };
 
+// HBM accesses of the fast kernels.  KFA_NT selects non-temporal (streaming) loads and
+// stores: every input is read once and every output written once.
+#ifndef KFA_NT
+#define KFA_NT 0
+#endif
+template <class St>
+__device__ __forceinline__ St hbm_ld(const St *p)
+{
+#if KFA_NT
+    if constexpr (sizeof(St) == 8) {
+        typedef unsigned u2 __attribute__((ext_vector_type(2)));
+        return __builtin_bit_cast(St, __builtin_nontemporal_load(reinterpret_cast<const u2 *>(p)));
+    } else if constexpr (sizeof(St) == 4) {
+        return __builtin_bit_cast(St, __builtin_nontemporal_load(reinterpret_cast<const unsigned *>(p)));
+    }
+#endif
+    return *p;
+}
+template <class St>
+__device__ __forceinline__ void hbm_st(St *p, St v)
+{
+#if KFA_NT
+    if constexpr (sizeof(St) == 8) {
+        typedef unsigned u2 __attribute__((ext_vector_type(2)));
+        __builtin_nontemporal_store(__builtin_bit_cast(u2, v), reinterpret_cast<u2 *>(p));
+        return;
+    } else if constexpr (sizeof(St) == 4) {
+        __builtin_nontemporal_store(__builtin_bit_cast(unsigned, v), reinterpret_cast<unsigned *>(p));
+        return;
+    }
+#endif
+    *p = v;
+}
+
 // Base pointers of global line gl: its segment's buffers and its index inside them.
 template <class S>
 __device__ __forceinline__ void line_base(const FastArgs &a, long long gl, const S *&in, S *&out)
@@ -330,7 +364,7 @@ struct FastStage {
             for (int r = 0; r < p; ++r) {
                 const int idx = u * P * p + r * P + o;
                 if (GSRC)
-                    v[j][r] = valid ? O::load(src[idx * src_el]) : O::zero();
+                    v[j][r] = valid ? O::load(hbm_ld(src + idx * src_el)) : O::zero();
                 else
                     v[j][r] = O::load(src[idx]);
             }
@@ -343,7 +377,7 @@ struct FastStage {
                                                         const St *__restrict__ rtw, int t, bool valid)
     {
         static_assert(m == 1, "the merge feeds the first stage");
-        auto X = [&](int k) { return valid ? O::load(bins[k * el]) : O::zero(); };
+        auto X = [&](int k) { return valid ? O::load(hbm_ld(bins + k * el)) : O::zero(); };
         if (t == 0) {
             // butterfly 0: z[r*NB], pairs (r, p-r); z[0] from DC and Nyquist
             {
@@ -466,7 +500,7 @@ struct FastStage {
                 const int idx = b + q * NB;
                 if (GDST) {
                     if (valid)
-                        dst[idx * dst_el] = O::pack(v[j][q]);
+                        hbm_st(dst + idx * dst_el, O::pack(v[j][q]));
                 } else {
                     dst[idx] = O::pack(v[j][q]);
                 }
@@ -481,7 +515,7 @@ struct FastStage {
     {
         if (!valid)
             return;
-        auto put = [&](int k, C c) { out[k * el] = O::pack(c); };
+        auto put = [&](int k, C c) { hbm_st(out + k * el, O::pack(c)); };
         if (t == 0) {
             {
                 C dc = O::fixdiv(v[0][0], 2);
@@ -606,14 +640,14 @@ __global__ void __launch_bounds__(T *LPB) k_fast(FastArgs a)
             if (k >= NK)
                 break;
             if (k == 0) {
-                C x0 = valid ? O::load(gin[0]) : O::zero();
-                C xn = valid ? O::load(gin[(long long)N * a.in_el]) : O::zero();
+                C x0 = valid ? O::load(hbm_ld(gin)) : O::zero();
+                C xn = valid ? O::load(hbm_ld(gin + (long long)N * a.in_el)) : O::zero();
                 C d{O::add(x0.r, xn.r), O::sub(x0.r, xn.r)};
                 bufA[0] = O::pack(O::fixdiv(d, 2));
                 continue;
             }
-            C xa = valid ? O::load(gin[(long long)k * a.in_el]) : O::zero();
-            C bz = valid ? O::load(gin[(long long)(N - k) * a.in_el]) : O::zero();
+            C xa = valid ? O::load(hbm_ld(gin + (long long)k * a.in_el)) : O::zero();
+            C bz = valid ? O::load(hbm_ld(gin + (long long)(N - k) * a.in_el)) : O::zero();
             C xb{bz.r, O::neg(bz.i)};
             xa = O::fixdiv(xa, 2);
             xb = O::fixdiv(xb, 2);
@@ -645,8 +679,8 @@ __global__ void __launch_bounds__(T *LPB) k_fast(FastArgs a)
                 break;
             if (k == 0) {
                 C dc = O::fixdiv(O::load(z[0]), 2);
-                gout[0] = O::pack(C{O::add(dc.r, dc.i), O::zero().r});
-                gout[(long long)N * a.out_el] = O::pack(C{O::sub(dc.r, dc.i), O::zero().r});
+                hbm_st(gout, O::pack(C{O::add(dc.r, dc.i), O::zero().r}));
+                hbm_st(gout + (long long)N * a.out_el, O::pack(C{O::sub(dc.r, dc.i), O::zero().r}));
                 continue;
             }
             C xa = O::load(z[k]);
@@ -657,8 +691,8 @@ __global__ void __launch_bounds__(T *LPB) k_fast(FastArgs a)
             C f1 = O::cadd(xa, xb), f2 = O::csub(xa, xb);
             C tw = O::cmul(f2, O::load(rtw[k - 1]));
             if (2 * k != N)
-                gout[(long long)k * a.out_el] = O::pack(C{O::half_add(f1.r, tw.r), O::half_add(f1.i, tw.i)});
-            gout[(long long)(N - k) * a.out_el] = O::pack(C{O::half_sub(f1.r, tw.r), O::half_sub(tw.i, f1.i)});
+                hbm_st(gout + (long long)k * a.out_el, O::pack(C{O::half_add(f1.r, tw.r), O::half_add(f1.i, tw.i)}));
+            hbm_st(gout + (long long)(N - k) * a.out_el, O::pack(C{O::half_sub(f1.r, tw.r), O::half_sub(tw.i, f1.i)}));
         }
     }
 }
