@@ -21,7 +21,7 @@ import numpy as np
 
 from . import build as _build
 
-__all__ = ["Library", "Server", "MultiBatch", "load", "SCALARS", "KissFFTError"]
+__all__ = ["Library", "Server", "MultiBatch", "FastFir", "load", "SCALARS", "KissFFTError"]
 
 SCALARS = {"float": np.float32, "int16_t": np.int16, "int32_t": np.int32}
 
@@ -151,6 +151,10 @@ class Library:
             "kiss_fft_multi_replay": ([vp, vp], ci),
             "kiss_fft_multi_info": ([vp, ctypes.POINTER(ll), ctypes.POINTER(ci), ctypes.POINTER(ci)], ci),
             "kiss_fft_multi_free": ([vp], None),
+            "kiss_fastfir_alloc_cpx": ([vp, ctypes.c_size_t, szp, vp, szp], vp),
+            "kiss_fastfir_alloc_real": ([vp, ctypes.c_size_t, szp, vp, szp], vp),
+            "kiss_fastfir_cpx": ([vp, vp, vp, ctypes.c_size_t, szp], ctypes.c_size_t),
+            "kiss_fastfir_real": ([vp, vp, vp, ctypes.c_size_t, szp], ctypes.c_size_t),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -278,9 +282,47 @@ class Library:
     def server(self, hbm_budget_bytes: int = 0, streams: int = 0, max_group: int = 0) -> "Server":
         return Server(self, hbm_budget_bytes, streams, max_group)
 
+    def fastfir_alloc(self, imp_resp, real: bool = False, nfft: int = 0) -> "FastFir":
+        return FastFir(self, imp_resp, real, nfft)
+
     def multi(self, items) -> "MultiBatch":
         """items: (kind, nfft, fin, fout, howmany) tuples, device buffers."""
         return MultiBatch(self, items)
+
+
+class FastFir:
+    """kiss_fastfir (include/kiss_fastfir.h): overlap-scrap FIR filter of a stream.
+
+    filter(inbuf, outbuf, n_new) follows the reference's kiss_fastfir: the first
+    offset + n_new samples of inbuf are filtered, the good output samples are written to
+    outbuf (their count is returned), and the unprocessed input is moved to the front of
+    inbuf (self.offset samples).  n_new == 0 flushes."""
+
+    def __init__(self, lib: Library, imp_resp, real: bool, nfft: int):
+        self.lib = lib
+        self.real = real
+        h = np.ascontiguousarray(imp_resp, dtype=lib.scalar)
+        n = ctypes.c_size_t(nfft)
+        nh = h.shape[0]
+        fn = lib.L.kiss_fastfir_alloc_real if real else lib.L.kiss_fastfir_alloc_cpx
+        self.ptr = fn(h.ctypes.data, nh, ctypes.byref(n), None, None)
+        if not self.ptr:
+            raise KissFFTError(f"kiss_fastfir_alloc failed: {lib.last_error()}")
+        self.nfft = n.value
+        self.ngood = self.nfft - nh + 1
+        self.offset = 0
+
+    def filter(self, inbuf, outbuf, n_new: int) -> int:
+        off = ctypes.c_size_t(self.offset)
+        fn = self.lib.L.kiss_fastfir_real if self.real else self.lib.L.kiss_fastfir_cpx
+        w = fn(self.ptr, _addr(inbuf), _addr(outbuf), n_new, ctypes.byref(off))
+        self.offset = off.value
+        return w
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and _libc is not None and getattr(_libc, "free", None) is not None:
+            _libc.free(self.ptr)
+            self.ptr = None
 
 
 class MultiItem(ctypes.Structure):

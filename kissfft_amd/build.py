@@ -37,8 +37,8 @@ DATATYPES = {
     "int16_t": ["-DFIXED_POINT=16"],
     "int32_t": ["-DFIXED_POINT=32"],
 }
-HOST_SOURCES = ["kf_core.c", "kf_real.c", "kf_nd.c", "kf_ndr.c", "kf_cache.c", "kf_serve.c", "kf_multi.c"]
-ENGINE_SOURCES = ["kfa_engine.hip", "kfa_fast_reg.hip", "kfa_tile_reg.hip", "kfa_serve.hip", "kfa_multi.hip"]
+HOST_SOURCES = ["kf_core.c", "kf_real.c", "kf_nd.c", "kf_ndr.c", "kf_cache.c", "kf_serve.c", "kf_multi.c", "kf_fastfir.c"]
+ENGINE_SOURCES = ["kfa_engine.hip", "kfa_fast_reg.hip", "kfa_tile_reg.hip", "kfa_serve.hip", "kfa_multi.hip", "kfa_conv.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CC = os.environ.get("CC", "gcc")

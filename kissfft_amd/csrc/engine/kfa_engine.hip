@@ -263,7 +263,7 @@ int build_rtab(const kfa_rplan *hp, int dev, std::shared_ptr<RealTables> &out)
 // ------------------------------------------------------------------------------------
 // fast variants: per-stage twiddle tables and first-use selection
 // ------------------------------------------------------------------------------------
-static bool env_flag(const char *name, bool dflt)
+bool env_flag(const char *name, bool dflt)
 {
     const char *v = getenv(name);
     if (!v || !*v)
@@ -274,7 +274,7 @@ static bool env_flag(const char *name, bool dflt)
 // Gathers the stage tables of schedule `rad` from the full-length table: row u of stage s
 // holds table[r*u*P_s] for r = 1..p-1, padded to a multiple of `pad` entries; then
 // table[P_s*m_s] and table[2*P_s*m_s], padded likewise (the layout of Radices<G, ...>).
-static int make_stage_table_raw(const unsigned char *h_tw, int type, long long n, const std::vector<int> &rad,
+int make_stage_table_raw(const unsigned char *h_tw, int type, long long n, const std::vector<int> &rad,
                                 int pad, int fmt, void **d_out)
 {
     // fmt 1: a Q15 entry w becomes the word pair (w.r, -w.i), (w.i, w.r) (8 bytes)
@@ -329,7 +329,7 @@ static int make_stage_table(const DevPlan &P, const std::vector<int> &rad, int p
 
 // Float twiddles of length n, the host library's expression (kiss_fft.c:361-367): phase
 // -2*pi*i/n in FP64 (negated for the inverse), cos / sin rounded once to float.
-static std::vector<unsigned char> float_twiddles(long long n, int inverse, long long count, long long step)
+std::vector<unsigned char> float_twiddles(long long n, int inverse, long long count, long long step)
 {
     const double pi = 3.14159265358979323846264338327;
     std::vector<unsigned char> h((size_t)count * 8);
@@ -797,14 +797,7 @@ int current_device(int *dev)
 }
 
 // A call's view of one operand: the caller's pointer, or a stream-ordered HBM copy.
-struct Operand {
-    const void *host = nullptr;  // non-null when staged
-    void *dev = nullptr;
-    size_t bytes = 0;
-    bool staged = false;
-};
-
-static int stage_in(const void *p, size_t bytes, hipStream_t s, Operand &op)
+int stage_in(const void *p, size_t bytes, hipStream_t s, Operand &op)
 {
     op.bytes = bytes;
     if (device_accessible(p)) {
@@ -821,7 +814,7 @@ static int stage_in(const void *p, size_t bytes, hipStream_t s, Operand &op)
     return 0;
 }
 
-static int stage_out(void *p, size_t bytes, hipStream_t s, Operand &op)
+int stage_out(void *p, size_t bytes, hipStream_t s, Operand &op)
 {
     op.bytes = bytes;
     if (device_accessible(p)) {
@@ -834,7 +827,7 @@ static int stage_out(void *p, size_t bytes, hipStream_t s, Operand &op)
     return 0;
 }
 
-static int finish_out(Operand &op, hipStream_t s)
+int finish_out(Operand &op, hipStream_t s)
 {
     if (op.staged && op.bytes) {
         KFA_HIP(hipMemcpyAsync(const_cast<void *>(op.host), op.dev, op.bytes, hipMemcpyDeviceToHost, s));
@@ -843,7 +836,7 @@ static int finish_out(Operand &op, hipStream_t s)
     return 0;
 }
 
-static void release(Operand &op, hipStream_t s)
+void release(Operand &op, hipStream_t s)
 {
     if (op.staged && op.dev)
         (void)hipFreeAsync(op.dev, s);
@@ -1504,6 +1497,7 @@ void kfa_cleanup(void)
     T.choice.clear();
     T.fs.clear();
     blue_release();
+    conv_release();
 }
 
 }  // extern "C"

@@ -74,3 +74,23 @@ struct BlueVariant {
 const std::vector<BlueVariant> &blue_registry();
 
 }  // namespace kfa
+
+namespace kfa {
+
+struct ConvArgs;
+
+// Fast-convolution variants (k_conv in kfa_blue.hpp): one block length N, float.
+struct ConvVariant {
+    int N;
+    int real;
+    int T, LPB;
+    std::vector<int> radices;
+    int tw_pad;
+    size_t lds;
+    const char *name;
+    int (*launch)(const ConvArgs &, hipStream_t);
+};
+
+const std::vector<ConvVariant> &conv_registry();
+
+}  // namespace kfa

@@ -62,6 +62,29 @@ int run_c2c(DevPlan &P, const void *in, long long istride, long long idist, void
 int run_real_rows(DevPlan &H, const RealTables &R, int inverse, const void *in, long long in_hi, void *out,
                   long long out_hi, long long howmany, hipStream_t s);
 
+// A caller buffer: used in place when the device can access it, else staged through a
+// stream-ordered device copy (stage_in copies host -> device; finish_out device -> host).
+struct Operand {
+    const void *host = nullptr;  // non-null when staged
+    void *dev = nullptr;
+    size_t bytes = 0;
+    bool staged = false;
+};
+int stage_in(const void *p, size_t bytes, hipStream_t s, Operand &op);
+int stage_out(void *p, size_t bytes, hipStream_t s, Operand &op);
+int finish_out(Operand &op, hipStream_t s);
+void release(Operand &op, hipStream_t s);
+
+bool env_flag(const char *name, bool dflt);
+// float twiddles exp(-+2*pi*i*(j*step mod n)/n), j < count, FP64 rounded once (8 bytes each)
+std::vector<unsigned char> float_twiddles(long long n, int inverse, long long count, long long step);
+// per-stage twiddle table of schedule `rad` (the fast kernels' layout) from a full table
+int make_stage_table_raw(const unsigned char *h_tw, int type, long long n, const std::vector<int> &rad, int pad,
+                         int fmt, void **d_out);
+
+// releases the fast-convolution device plans (kfa_conv.hip)
+void conv_release();
+
 // entry / exit of a C-ABI call: the stream to use (the caller's, or the calling thread's
 // own blocking stream for NULL) and the optional synchronisation at the end
 int call_begin(void *stream, hipStream_t *s, int *dev);

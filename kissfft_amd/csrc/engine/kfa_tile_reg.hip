@@ -129,4 +129,54 @@ const std::vector<BlueVariant> &blue_registry()
     return *reg;
 }
 
+template <int T, int LPB, bool REAL, class RS>
+static int launch_conv(const ConvArgs &a, hipStream_t s)
+{
+    const size_t lds = (size_t)LPB * (RS::N() + 1) * sizeof(float2);
+    static std::once_flag once;
+    std::call_once(once, [=] {
+        (void)hipFuncSetAttribute((const void *)k_conv<T, LPB, REAL, RS>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds);
+    });
+    const long long blocks = (a.nlines + LPB - 1) / LPB;
+    if (blocks <= 0)
+        return 0;
+    hipLaunchKernelGGL((k_conv<T, LPB, REAL, RS>), dim3((unsigned)blocks), dim3(T * LPB), lds, s, a);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+template <int T, int LPB, int... R>
+static void add_conv(std::vector<ConvVariant> &v, const char *name)
+{
+    typedef Radices<2, R...> RS;
+    for (int real = 0; real < 2; ++real) {
+        ConvVariant f;
+        f.N = RS::N();
+        f.real = real;
+        f.T = T;
+        f.LPB = LPB;
+        f.radices = {R...};
+        f.tw_pad = 2;
+        f.lds = (size_t)LPB * (RS::N() + 1) * 8;
+        f.name = name;
+        f.launch = real ? &launch_conv<T, LPB, true, RS> : &launch_conv<T, LPB, false, RS>;
+        v.push_back(f);
+    }
+}
+
+const std::vector<ConvVariant> &conv_registry()
+{
+    static std::vector<ConvVariant> *reg = [] {
+        auto *v = new std::vector<ConvVariant>();
+        add_conv<16, 4, 16, 16>(*v, "conv256_1616_t16x4");
+        add_conv<64, 2, 8, 8, 8>(*v, "conv512_888_t64x2");
+        add_conv<64, 2, 16, 16, 4>(*v, "conv1024_16164_t64x2");
+        add_conv<128, 2, 16, 16, 8>(*v, "conv2048_16168_t128x2");
+        add_conv<256, 1, 16, 16, 16>(*v, "conv4096_161616_t256");
+        add_conv<256, 1, 16, 16, 8, 4>(*v, "conv8192_161684_t256");
+        return v;
+    }();
+    return *reg;
+}
+
 }  // namespace kfa

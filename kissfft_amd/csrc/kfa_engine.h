@@ -118,6 +118,11 @@ KFA_HIDDEN int kfa_multi_replay(void *m, void *stream, int sync);
 KFA_HIDDEN int kfa_multi_info(void *m, long long *tiles, int *grid, int *persistent_items);
 KFA_HIDDEN void kfa_multi_destroy(void *m);
 
+/* Fast FIR by overlap-scrap: nblocks windows in[k*ngood .. +nfft) -> out[k*ngood .. +ngood).
+ * H: nfft complex (host), the filter spectrum scaled by 1/nfft.  real: samples are scalars. */
+KFA_HIDDEN int kfa_fastfir(int type, int nfft, int real, const void *H, const void *in, void *out, long long nblocks,
+                           long long ngood, void *stream, int sync);
+
 #ifdef __cplusplus
 }
 #endif

@@ -165,3 +165,28 @@ def test_no_gpu_fails_loudly():
     with pytest.raises(kissfft_amd.KissFFTError, match="no HIP device"):
         lib.fft_batch(cfg, x, y, 1, 64, 64)
     assert np.all(y == 7.0)  # nothing written, no CPU fallback
+
+
+def test_fastfir_header_maps_real_names(tmp_path):
+    """REAL_FASTFIR selects the real filter under the reference's names (the header's
+    macro mapping), both spellings compile against the declarations."""
+    import subprocess
+    src = tmp_path / "ff.c"
+    src.write_text('#include "kiss_fastfir.h"\n'
+                   "size_t f(kiss_fft_scalar *h, kiss_fft_scalar *x, kiss_fft_scalar *y) {\n"
+                   "  size_t nfft = 0, off = 0;\n"
+                   "  kiss_fastfir_cfg c = kiss_fastfir_alloc(h, 10, &nfft, 0, 0);\n"
+                   "  return kiss_fastfir(c, x, y, 100, &off); }\n")
+    inc = os.path.join(ROOT, "include")
+    r = subprocess.run(["gcc", "-fsyntax-only", "-Werror", "-DREAL_FASTFIR", "-I", inc, str(src)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run(["gcc", "-fsyntax-only", "-Werror", "-I", inc, str(src)], capture_output=True, text=True)
+    assert r.returncode != 0  # without REAL_FASTFIR the names take complex samples
+
+
+def test_fastfir_fixed_point_refused():
+    lib = kissfft_amd.load("int16_t")
+    h = np.ones(8, np.int16)
+    with pytest.raises(kissfft_amd.KissFFTError):
+        lib.fastfir_alloc(h, real=True)
