@@ -190,3 +190,11 @@ def test_fastfir_fixed_point_refused():
     h = np.ones(8, np.int16)
     with pytest.raises(kissfft_amd.KissFFTError):
         lib.fastfir_alloc(h, real=True)
+
+
+def test_cpp_class_header_compiles():
+    """include/kissfft.hh compiles as C++17 with warnings as errors (the test driver)."""
+    import subprocess
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                        os.path.join(ROOT, "tests", "cpp", "facade_driver.cpp")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
