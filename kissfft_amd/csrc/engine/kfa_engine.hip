@@ -364,7 +364,74 @@ static TileCache &tile_cache()
 }
 
 // The tile variant for (n, inverse, mode) on the current device, or v == nullptr.
-static int tile_choice(int n, int inverse, int mode, const TileChoice **out)
+static int launch_tile(const TileChoice &tc, TileArgs a, hipStream_t s);
+
+// Times the tile candidates of one (length, direction, mapping) on a scratch layout of
+// that mapping (1024 lines per group, about 256 MiB moved per launch) and returns the
+// index of the fastest; interleaved rounds as for the line kernels.
+static int tune_tiles(int n, int mode, std::vector<TileChoice> &cand, hipStream_t s, size_t *best)
+{
+    const long long C = 1024;
+    const long long outer = std::max(1LL, (16LL << 20) / ((long long)n * C));
+    const size_t bytes = 8 * (size_t)n * (size_t)C * (size_t)outer;
+    void *bin = nullptr, *bout = nullptr, *whi = nullptr, *wlo = nullptr;
+    const size_t hib = 8 * (size_t)((long long)n * C / 1024 + 1);
+    KFA_HIP(scratch_alloc(&bin, bytes, s));
+    KFA_HIP(scratch_alloc(&bout, bytes, s));
+    KFA_HIP(scratch_alloc(&whi, hib, s));
+    KFA_HIP(scratch_alloc(&wlo, 8 * 1024, s));
+    KFA_HIP(hipMemsetAsync(bin, 0x3c, bytes, s));
+    KFA_HIP(hipMemsetAsync(whi, 0x3c, hib, s));
+    KFA_HIP(hipMemsetAsync(wlo, 0x3c, 8 * 1024, s));
+    TileArgs a{};
+    a.in = bin;
+    a.out = bout;
+    a.tw_hi = whi;
+    a.tw_lo = wlo;
+    a.nlines = outer * C;
+    a.nlo = C;
+    a.in_hi = a.out_hi = (long long)n * C;
+    a.out_lo = 1;
+    a.out_el = C;
+    a.in_lo = mode == TILE_ROWS_TO_COLS ? n : 1;
+    a.in_el = mode == TILE_ROWS_TO_COLS ? 1 : C;
+    hipEvent_t e0, e1;
+    KFA_HIP(hipEventCreate(&e0));
+    KFA_HIP(hipEventCreate(&e1));
+    std::vector<float> t(cand.size(), 1e30f);
+    int rc = 0;
+    for (size_t i = 0; i < cand.size() && rc == 0; ++i)
+        rc = launch_tile(cand[i], a, s);  // warm-up
+    for (int round = 0; round < 3 && rc == 0; ++round)
+        for (size_t i = 0; i < cand.size() && rc == 0; ++i) {
+            KFA_HIP(hipEventRecord(e0, s));
+            for (int r = 0; r < 3 && rc == 0; ++r)
+                rc = launch_tile(cand[i], a, s);
+            KFA_HIP(hipEventRecord(e1, s));
+            KFA_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            KFA_HIP(hipEventElapsedTime(&ms, e0, e1));
+            t[i] = std::min(t[i], ms);
+        }
+    *best = 0;
+    for (size_t i = 1; i < cand.size(); ++i)
+        if (t[i] < 0.98f * t[*best])
+            *best = i;
+    if (env_flag("KISSFFT_AMD_VERBOSE", false))
+        for (size_t i = 0; i < cand.size(); ++i)
+            fprintf(stderr, "[kissfft_amd]   %-32s mode %d %8.1f us\n", cand[i].v->name, mode, 1e3f * t[i] / 3);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    for (void *p : {bin, bout, whi, wlo})
+        (void)hipFreeAsync(p, s);
+    KFA_HIP(hipStreamSynchronize(s));
+    return rc;
+}
+
+// The tile variant for (n, inverse, mode) on the current device, or v == nullptr; several
+// candidates are timed once (KISSFFT_AMD_TILE forces one by name, KISSFFT_AMD_TUNE=0
+// takes the first registered).
+static int tile_choice(int n, int inverse, int mode, hipStream_t s, const TileChoice **out)
 {
     int dev;
     int rc = current_device(&dev);
@@ -378,24 +445,35 @@ static int tile_choice(int n, int inverse, int mode, const TileChoice **out)
         *out = &it->second;
         return 0;
     }
-    TileChoice tc;
+    std::vector<TileChoice> cand;
     if (env_flag("KISSFFT_AMD_FAST", true)) {
         const char *force = getenv("KISSFFT_AMD_TILE");
-        for (const TileVariant &v : tile_registry()) {
-            if (v.n != n || v.inverse != inverse || v.mode != mode)
-                continue;
-            if (!tc.v || (force && strcmp(force, v.name) == 0))
-                tc.v = &v;
-        }
+        for (const TileVariant &v : tile_registry())
+            if (v.n == n && v.inverse == inverse && v.mode == mode && (!force || !*force || !strcmp(force, v.name)))
+                cand.push_back(TileChoice{&v, nullptr});
+        if (cand.empty() && force && *force)  // forced name of another length: all candidates
+            for (const TileVariant &v : tile_registry())
+                if (v.n == n && v.inverse == inverse && v.mode == mode) {
+                    cand.push_back(TileChoice{&v, nullptr});
+                    break;
+                }
     }
-    if (tc.v) {
-        const std::vector<unsigned char> tw = float_twiddles(n, inverse, n, 1);
-        rc = make_stage_table_raw(tw.data(), KFA_F32, n, tc.v->radices, tc.v->tw_pad, 0, &tc.d_stw);
-        if (rc)
+    const std::vector<unsigned char> tw = float_twiddles(n, inverse, n, 1);
+    for (TileChoice &c : cand)
+        if ((rc = make_stage_table_raw(tw.data(), KFA_F32, n, c.v->radices, c.v->tw_pad, 0, &c.d_stw)))
             return rc;
-        if (env_flag("KISSFFT_AMD_VERBOSE", false))
-            fprintf(stderr, "[kissfft_amd] tile n=%d inverse=%d mode=%d: %s\n", n, inverse, mode, tc.v->name);
+    size_t best = 0;
+    if (cand.size() > 1 && env_flag("KISSFFT_AMD_TUNE", true) && (rc = tune_tiles(n, mode, cand, s, &best)))
+        return rc;
+    TileChoice tc;
+    for (size_t i = 0; i < cand.size(); ++i) {
+        if (i == best)
+            tc = cand[i];
+        else
+            (void)hipFree(cand[i].d_stw);
     }
+    if (tc.v && env_flag("KISSFFT_AMD_VERBOSE", false))
+        fprintf(stderr, "[kissfft_amd] tile n=%d inverse=%d mode=%d: %s\n", n, inverse, mode, tc.v->name);
     auto &slot = C.choice[key];
     slot = tc;
     *out = &slot;
@@ -600,8 +678,8 @@ static int run_c2c_bailey(DevPlan &P, const void *in, long long istride, long lo
         return 1;
     const long long N1 = best1, N2 = n / N1;
     int rc;
-    if ((rc = tile_choice((int)N1, P.inverse, TILE_ROWS_TO_COLS, &c2)) ||
-        (rc = tile_choice((int)N2, P.inverse, TILE_COLS_TWID, &c1)))
+    if ((rc = tile_choice((int)N1, P.inverse, TILE_ROWS_TO_COLS, s, &c2)) ||
+        (rc = tile_choice((int)N2, P.inverse, TILE_COLS_TWID, s, &c1)))
         return rc;
     if (!c1->v || !c2->v)
         return 1;
@@ -1305,7 +1383,7 @@ static int run_axis(DevPlan &P, const void *in, void *out, long long outer, long
         return fail(-2, "N-D axis of %lld points with inner stride %lld: not yet supported", len, inner);
     if (P.type == KFA_F32) {
         const TileChoice *tc = nullptr;
-        int rc = tile_choice((int)len, P.inverse, TILE_COLS, &tc);
+        int rc = tile_choice((int)len, P.inverse, TILE_COLS, s, &tc);
         if (rc)
             return rc;
         if (tc->v) {

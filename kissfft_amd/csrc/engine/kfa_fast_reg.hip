@@ -139,6 +139,13 @@ const std::vector<FastVariant> &fast_registry()
         add_variant<F32, FK_C2C, 64, 4, false, false, 4, 4, 4, 4>(*v, KFA_F32, "c2c256_4444_t64x4_sb");
         add_variant<F32, FK_C2C, 100, 2, false, false, 5, 5, 5, 4, 2>(*v, KFA_F32, "c2c1000_55542_t100x2_sb");
         add_variant<F32, FK_C2C, 125, 2, false, false, 5, 5, 5, 8>(*v, KFA_F32, "c2c1000_5558_t125x2_sb");
+        // N-D axis lengths (contiguous last axes; the non-last axes run as tiles)
+        add_variant<F32, FK_C2C, 40, 8, false, false, 8, 3, 5>(*v, KFA_F32, "c2c120_835_t40x8_sb");
+        add_variant<F32, FK_C2C, 24, 8, false, false, 8, 5, 3>(*v, KFA_F32, "c2c120_853_t24x8_sb");
+        add_variant<F32, FK_C2C, 16, 16, false, false, 8, 4, 3>(*v, KFA_F32, "c2c96_843_t16x16_sb");
+        add_variant<F32, FK_C2C, 8, 32, false, false, 8, 8>(*v, KFA_F32, "c2c64_88_t8x32_sb");
+        add_variant<F32, FK_C2C, 16, 16, false, false, 8, 16>(*v, KFA_F32, "c2c128_816_t16x16_sb");
+        add_variant<F32, FK_C2C, 125, 2, false, false, 5, 5, 5, 4>(*v, KFA_F32, "c2c500_5554_t125x2_sb");
         add_variant<F32, FK_C2C, 192, 1, false, false, 3, 16, 8, 8>(*v, KFA_F32, "c2c3072_31688_t192_sb");
         add_variant<F32, FK_C2C, 256, 1, false, false, 16, 16, 16>(*v, KFA_F32, "c2c4096_161616_t256_sb");
         // ---- Q15 (reference schedules), packed-word butterflies ----
