@@ -720,7 +720,7 @@ static int run_c2c_bailey(DevPlan &P, const void *in, long long istride, long lo
 }
 
 // Chooses (once per plan and kind) the fastest registered variant, or none.
-int choose_fast(DevPlan &P, int kind, hipStream_t s, const FastChoice **out)
+int choose_fast(DevPlan &P, int kind, hipStream_t s, const FastChoice **out, long long lines_hint)
 {
     std::lock_guard<std::mutex> g(P.mu);
     FastChoice &fc = P.fast[kind];
@@ -760,12 +760,13 @@ int choose_fast(DevPlan &P, int kind, hipStream_t s, const FastChoice **out)
     }
     if (!forced && cand.size() > 1 && env_flag("KISSFFT_AMD_TUNE", true)) {
         // time every candidate on a scratch batch large enough to stream from HBM
-        // (about 512 MiB in + out, beyond the last-level cache)
+        // as large as the call that asks (between 256 MiB and 1 GiB per side, beyond the
+        // last-level cache): the ranking of variants depends on the batch size
         const size_t cb = cpx_bytes(P.type);
-        long long lines = (256LL << 20) / ((long long)cb * (P.n + 1));
-        if (lines < 64)
-            lines = 64;
-        const size_t bytes = cb * (size_t)(P.n + 1) * (size_t)lines * 2;
+        const long long per_line = (long long)cb * (P.n + 1);
+        long long lines = std::max(lines_hint, (256LL << 20) / per_line);
+        lines = std::max(64LL, std::min(lines, (1LL << 30) / per_line));
+        const size_t bytes = (size_t)per_line * (size_t)lines;
         void *bin = nullptr, *bout = nullptr, *brtw = nullptr;
         KFA_HIP(scratch_alloc(&bin, bytes, s));
         KFA_HIP(scratch_alloc(&bout, bytes, s));
@@ -807,9 +808,10 @@ int choose_fast(DevPlan &P, int kind, hipStream_t s, const FastChoice **out)
                 t_best[i] = std::min(t_best[i], ms);
             }
         }
-        // a later candidate must win by 2%: registration order breaks near-ties
+        // a later candidate must win by 4%: registration order (fastest in full-size
+        // sweeps first) breaks near-ties, so the choice does not flip with timing noise
         for (size_t i = 1; i < cand.size(); ++i)
-            if (t_best[i] < 0.98f * t_best[best])
+            if (t_best[i] < 0.96f * t_best[best])
                 best = i;
         if (env_flag("KISSFFT_AMD_VERBOSE", false))
             for (size_t i = 0; i < cand.size(); ++i)
@@ -1016,7 +1018,7 @@ int run_c2c(DevPlan &P, const void *in, long long istride, long long idist, void
     }
     if (P.single) {
         const FastChoice *fc = nullptr;
-        int rc = choose_fast(P, FK_C2C, s, &fc);
+        int rc = choose_fast(P, FK_C2C, s, &fc, howmany);
         if (rc)
             return rc;
         if (fc->v) {
@@ -1183,7 +1185,7 @@ int run_real_rows(DevPlan &H, const RealTables &R, int inverse, const void *in, 
     if (!H.single)
         return run_real_multipass(H, R, inverse, in, in_hi, out, out_hi, howmany, s);
     const FastChoice *fc = nullptr;
-    int rc = choose_fast(H, inverse ? FK_C2R : FK_R2C, s, &fc);
+    int rc = choose_fast(H, inverse ? FK_C2R : FK_R2C, s, &fc, howmany);
     if (rc)
         return rc;
     if (fc->v) {

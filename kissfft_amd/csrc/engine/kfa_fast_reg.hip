@@ -5,7 +5,8 @@
 // reference radix schedule of their length (kf_factor order, executed innermost first) and
 // are used only when the plan's schedule equals it; float variants may use any schedule.
 // When several variants serve the same key the engine times them on first use and keeps
-// the fastest.
+// the fastest; a later variant must win by a margin, so the first registered of a key --
+// the fastest in full-size sweeps -- is kept on near-ties.
 #include <hip/hip_runtime.h>
 
 #include <mutex>
@@ -97,18 +98,18 @@ const std::vector<FastVariant> &fast_registry()
         auto *v = new std::vector<FastVariant>();
         // ---- float real transforms; the line is the half length ----
         // N = 4096 (half 2048)
-        add_real<F32, 256, 1, true, 8, 8, 8, 4>(*v, KFA_F32, "real2048_8884_t256_db");
-        add_real<F32, 256, 1, false, 8, 8, 8, 4>(*v, KFA_F32, "real2048_8884_t256_sb");
         add_real<F32, 128, 1, false, 16, 16, 8>(*v, KFA_F32, "real2048_16168_t128_sb");
+        add_real<F32, 256, 1, false, 8, 8, 8, 4>(*v, KFA_F32, "real2048_8884_t256_sb");
+        add_real<F32, 256, 1, true, 8, 8, 8, 4>(*v, KFA_F32, "real2048_8884_t256_db");
         add_real<F32, 128, 1, true, 16, 16, 8>(*v, KFA_F32, "real2048_16168_t128_db");
         add_real<F32, 128, 1, false, 8, 16, 16>(*v, KFA_F32, "real2048_81616_t128_sb");
         add_real<F32, 128, 2, false, 16, 16, 8>(*v, KFA_F32, "real2048_16168_t128x2_sb");
         // N = 4000 (half 2000)
+        add_real<F32, 250, 1, false, 8, 2, 5, 5, 5>(*v, KFA_F32, "real2000_82555_t250_sb");
         add_real<F32, 100, 2, true, 4, 4, 5, 5, 5>(*v, KFA_F32, "real2000_44555_t100x2_db");
         add_real<F32, 100, 2, false, 4, 4, 5, 5, 5>(*v, KFA_F32, "real2000_44555_t100x2_sb");
         add_real<F32, 250, 1, false, 5, 5, 5, 4, 4>(*v, KFA_F32, "real2000_55544_t250_sb");
         add_real<F32, 125, 2, false, 16, 5, 5, 5>(*v, KFA_F32, "real2000_16555_t125x2_sb");
-        add_real<F32, 250, 1, false, 8, 2, 5, 5, 5>(*v, KFA_F32, "real2000_82555_t250_sb");
         add_fused<F32, FK_R2C, 128, 1, 16, 16, 8>(*v, KFA_F32, "r2c2048_16168_t128_fused");
         add_fused<F32, FK_C2R, 128, 1, 8, 16, 16>(*v, KFA_F32, "c2r2048_81616_t128_fused");
         add_fused<F32, FK_R2C, 128, 2, 16, 16, 8>(*v, KFA_F32, "r2c2048_16168_t128x2_fused");
@@ -127,8 +128,8 @@ const std::vector<FastVariant> &fast_registry()
         add_fused<F32, FK_R2C, 50, 4, 4, 5, 5, 5>(*v, KFA_F32, "r2c500_4555_t50x4_fused");
         add_fused<F32, FK_C2R, 50, 4, 5, 5, 5, 4>(*v, KFA_F32, "c2r500_5554_t50x4_fused");
         // ---- float complex ----
-        add_variant<F32, FK_C2C, 240, 1, false, false, 4, 4, 3, 5, 5>(*v, KFA_F32, "c2c1200_44355_t240_sb");
         add_variant<F32, FK_C2C, 240, 1, true, false, 4, 4, 3, 5, 5>(*v, KFA_F32, "c2c1200_44355_t240_db");
+        add_variant<F32, FK_C2C, 240, 1, false, false, 4, 4, 3, 5, 5>(*v, KFA_F32, "c2c1200_44355_t240_sb");
         add_variant<F32, FK_C2C, 120, 2, false, false, 5, 5, 3, 16>(*v, KFA_F32, "c2c1200_55316_t120x2_sb");
         add_variant<F32, FK_C2C, 128, 1, false, false, 16, 16, 8>(*v, KFA_F32, "c2c2048_16168_t128_sb");
         add_variant<F32, FK_C2C, 256, 1, false, false, 8, 8, 8, 4>(*v, KFA_F32, "c2c2048_8884_t256_sb");
@@ -137,8 +138,8 @@ const std::vector<FastVariant> &fast_registry()
         add_variant<F32, FK_C2C, 64, 4, false, false, 8, 8, 8>(*v, KFA_F32, "c2c512_888_t64x4_sb");
         add_variant<F32, FK_C2C, 16, 16, false, false, 16, 16>(*v, KFA_F32, "c2c256_1616_t16x16_sb");
         add_variant<F32, FK_C2C, 64, 4, false, false, 4, 4, 4, 4>(*v, KFA_F32, "c2c256_4444_t64x4_sb");
-        add_variant<F32, FK_C2C, 100, 2, false, false, 5, 5, 5, 4, 2>(*v, KFA_F32, "c2c1000_55542_t100x2_sb");
         add_variant<F32, FK_C2C, 125, 2, false, false, 5, 5, 5, 8>(*v, KFA_F32, "c2c1000_5558_t125x2_sb");
+        add_variant<F32, FK_C2C, 100, 2, false, false, 5, 5, 5, 4, 2>(*v, KFA_F32, "c2c1000_55542_t100x2_sb");
         // N-D axis lengths (contiguous last axes; the non-last axes run as tiles)
         add_variant<F32, FK_C2C, 40, 8, false, false, 8, 3, 5>(*v, KFA_F32, "c2c120_835_t40x8_sb");
         add_variant<F32, FK_C2C, 24, 8, false, false, 8, 5, 3>(*v, KFA_F32, "c2c120_853_t24x8_sb");
@@ -149,10 +150,10 @@ const std::vector<FastVariant> &fast_registry()
         add_variant<F32, FK_C2C, 192, 1, false, false, 3, 16, 8, 8>(*v, KFA_F32, "c2c3072_31688_t192_sb");
         add_variant<F32, FK_C2C, 256, 1, false, false, 16, 16, 16>(*v, KFA_F32, "c2c4096_161616_t256_sb");
         // ---- Q15 (reference schedules), packed-word butterflies ----
+        add_variant<Q15P, FK_C2C, 64, 4, true, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c512_t64x4_db");
         add_variant<Q15P, FK_C2C, 64, 4, false, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c512_t64x4_sb");
         add_variant<Q15P, FK_C2C, 128, 2, false, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c512_t128x2_sb");
         add_variant<Q15P, FK_C2C, 32, 8, false, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c512_t32x8_sb");
-        add_variant<Q15P, FK_C2C, 64, 4, true, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c512_t64x4_db");
         add_variant<Q15P, FK_C2C, 240, 1, false, false, 5, 5, 3, 4, 4>(*v, KFA_Q15, "q15p_c2c1200_t240_sb");
         add_variant<Q15P, FK_C2C, 64, 4, false, false, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c256_t64x4_sb");
         add_variant<Q15P, FK_C2C, 128, 2, false, false, 4, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c1024_t128x2_sb");

@@ -53,8 +53,9 @@ bool device_accessible(const void *p);
 // device plans (cached; thread-safe)
 int build_cplan(const kfa_cplan *hp, int dev, std::shared_ptr<DevPlan> &out);
 int build_rtab(const kfa_rplan *hp, int dev, std::shared_ptr<RealTables> &out);
-// the fast variant for one transform kind of a plan (decided once per plan and kind)
-int choose_fast(DevPlan &P, int kind, hipStream_t s, const FastChoice **out);
+// the fast variant for one transform kind of a plan (decided once per plan and kind, timed
+// on a batch sized from lines_hint, the number of lines of the call that asks)
+int choose_fast(DevPlan &P, int kind, hipStream_t s, const FastChoice **out, long long lines_hint = 0);
 
 // launches between device buffers, enqueued on s
 int run_c2c(DevPlan &P, const void *in, long long istride, long long idist, void *out, long long ostride,

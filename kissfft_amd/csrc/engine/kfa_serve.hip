@@ -371,7 +371,7 @@ long long kfa_server_submit(void *srv, int kind, const kfa_cplan *cp, const kfa_
     if (r->P->single) {
         const FastChoice *fc;
         const int fk = kind == KFA_SERVE_R2C ? FK_R2C : kind == KFA_SERVE_C2R ? FK_C2R : FK_C2C;
-        if ((rc = choose_fast(*r->P, fk, S->plan_stream, &fc)))
+        if ((rc = choose_fast(*r->P, fk, S->plan_stream, &fc, howmany)))
             return rc;
     }
     side_bytes(S->type, kind, nfft, howmany, r->in_bytes, r->out_bytes);
