@@ -17,7 +17,8 @@
 // external or hierarchical memory", 1990):  with n = n1 + N1*n2 and k = k2 + N2*k1,
 //   X[k2 + N2*k1] = sum_n1 W_N1^(n1*k1) * W_N^(n1*k2) * sum_n2 x[n1 + N1*n2] W_N2^(n2*k2).
 // W^a is read as hi[a >> 10] * lo[a & 1023] from two small tables (computed in FP64,
-// rounded once).  The float path only: fixed-point long lines keep the reference's own
+// rounded once); a thread's IT elements share one such product and a per-line table of
+// W^(lo*T*k) in LDS.  The float path only: fixed-point long lines keep the reference's own
 // stage order (the general line kernel's two-pass split).
 #pragma once
 
@@ -77,8 +78,12 @@ __global__ void __launch_bounds__(T *TL) k_tile(TileArgs a)
     constexpr int NS = N + 1;
     constexpr int IT = N / T;  // elements per thread in the HBM phases
     static_assert(N % T == 0, "threads per line must divide the length");
+    static_assert(!TWID || SCOL, "the four-step twiddle is applied on column stores");
     extern __shared__ __align__(16) unsigned char kfa_tile_smem[];
     St *buf = reinterpret_cast<St *>(kfa_tile_smem);
+    // TWID: per line, W^(lo*T*k) for k < IT; a thread's element e = e0 + k*T then takes
+    // W^(lo*e) = W^(lo*e0) * W^(lo*T*k) (one table product per thread, one per line and k)
+    __shared__ St s_step[TWID ? TL * IT : 1];
 
     const int tid = threadIdx.x;
     const long long ntiles = (a.nlines + TL - 1) / TL;
@@ -108,6 +113,17 @@ __global__ void __launch_bounds__(T *TL) k_tile(TileArgs a)
             if (ok)
                 buf[ln * NS + e0 + k * T] = v[k];
     }
+    if constexpr (TWID) {
+        const St *whi = reinterpret_cast<const St *>(a.tw_hi);
+        const St *wlo = reinterpret_cast<const St *>(a.tw_lo);
+        for (int w = tid; w < TL * IT; w += T * TL) {
+            const int ln = w / IT, k = w - (w / IT) * IT;
+            const long long gl = l0 + (ln < nl ? ln : 0);
+            const unsigned long long lo = (unsigned long long)(gl % a.nlo);
+            const unsigned long long ex = lo * (unsigned)(k * T);  // < tw_n
+            s_step[w] = O::pack(O::cmul(O::load(whi[ex >> 10]), O::load(wlo[ex & 1023])));
+        }
+    }
     __syncthreads();
 
     {
@@ -131,16 +147,11 @@ __global__ void __launch_bounds__(T *TL) k_tile(TileArgs a)
         if constexpr (TWID) {
             const St *whi = reinterpret_cast<const St *>(a.tw_hi);
             const St *wlo = reinterpret_cast<const St *>(a.tw_lo);
-            C wh[IT], wl[IT];
-#pragma unroll
-            for (int k = 0; k < IT; ++k) {
-                const unsigned long long ex = (unsigned long long)lo * (unsigned)(e0 + k * T);  // < tw_n
-                wh[k] = O::load(whi[ex >> 10]);
-                wl[k] = O::load(wlo[ex & 1023]);
-            }
+            const unsigned long long ex = (unsigned long long)lo * (unsigned)e0;  // < tw_n
+            const C base = O::cmul(O::load(whi[ex >> 10]), O::load(wlo[ex & 1023]));
 #pragma unroll
             for (int k = 0; k < IT; ++k)
-                v[k] = O::cmul(v[k], O::cmul(wh[k], wl[k]));
+                v[k] = O::cmul(v[k], O::cmul(base, O::load(s_step[ln * IT + k])));
         }
 #pragma unroll
         for (int k = 0; k < IT; ++k)
