@@ -49,6 +49,13 @@ kiss_fft_server KISS_FFT_API kiss_fft_server_create(long long hbm_budget_bytes, 
 long long KISS_FFT_API kiss_fft_server_submit(kiss_fft_server srv, int kind, int nfft, const void *in, void *out,
                                               long long howmany);
 
+/* Queues one N-D request: `howmany` arrays of dims[0] x ... x dims[ndims-1] (dims[0]
+ * slowest, as kiss_fftnd / kiss_fftndr).  Kinds: complex forward / inverse, real forward
+ * (last dimension even; its nfft/2+1 bins on the output side) / real inverse.  N-D
+ * requests are admitted, staged and overlapped like 1-D ones and launched one by one. */
+long long KISS_FFT_API kiss_fft_server_submit_nd(kiss_fft_server srv, int kind, int ndims, const int *dims,
+                                                 const void *in, void *out, long long howmany);
+
 /* Blocks until the request is done (0) or failed (< 0); releases the ticket. */
 int KISS_FFT_API kiss_fft_server_wait(kiss_fft_server srv, long long ticket, double *latency_s);
 

@@ -140,6 +140,7 @@ class Library:
             "kiss_fft_get_engine_stats": ([vp], None),
             "kiss_fft_server_create": ([ll, ci, ci], vp),
             "kiss_fft_server_submit": ([vp, ci, ci, vp, vp, ll], ll),
+            "kiss_fft_server_submit_nd": ([vp, ci, ci, ip, vp, vp, ll], ll),
             "kiss_fft_server_wait": ([vp, ll, ctypes.POINTER(ctypes.c_double)], ci),
             "kiss_fft_server_drain": ([vp], ci),
             "kiss_fft_server_get_stats": ([vp, ctypes.POINTER(ServerStats)], ci),
@@ -403,6 +404,13 @@ class Server:
         t = self.lib.L.kiss_fft_server_submit(self.ptr, kind, nfft, _addr(fin), _addr(fout), howmany)
         if t <= 0:
             raise KissFFTError(f"kiss_fft_server_submit failed ({t}): {self.lib.last_error()}")
+        return t
+
+    def submit_nd(self, kind: int, dims, fin, fout, howmany: int) -> int:
+        d = (ctypes.c_int * len(dims))(*dims)
+        t = self.lib.L.kiss_fft_server_submit_nd(self.ptr, kind, len(dims), d, _addr(fin), _addr(fout), howmany)
+        if t <= 0:
+            raise KissFFTError(f"kiss_fft_server_submit_nd failed ({t}): {self.lib.last_error()}")
         return t
 
     def wait(self, ticket: int) -> float:

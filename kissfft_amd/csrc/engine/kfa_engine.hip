@@ -1420,6 +1420,78 @@ static int run_axis(DevPlan &P, const void *in, void *out, long long outer, long
     return launch_lines(P.type, ll, s);
 }
 
+}  // extern "C" (reopened below)
+
+namespace kfa {
+
+// Complex N-D transform of `howmany` arrays between device buffers: axis 0 first, as
+// kiss_fftnd.c:181-186 does (fixed point rounds per axis, so the order is part of the
+// result); the first axis reads the input, later axes run in place on the output.
+int run_nd(const std::vector<std::shared_ptr<DevPlan>> &P, const int *dims, const void *in, void *out,
+           long long howmany, hipStream_t s)
+{
+    const int ndims = (int)P.size();
+    long long total = 1;
+    for (int k = 0; k < ndims; ++k)
+        total *= dims[k];
+    long long outer = howmany, inner = total;
+    const void *src = in;
+    int rc = 0;
+    for (int k = 0; k < ndims && rc == 0; ++k) {
+        inner /= dims[k];
+        rc = run_axis(*P[k], src, out, outer, inner, s);
+        outer *= dims[k];
+        src = out;
+    }
+    return rc;
+}
+
+// Real N-D transform between device buffers (kiss_fftndr.c:99-109 forward: real rows,
+// then the complex axes per bin column; kiss_fftndr.c:121-131 inverse: the complex axes
+// into a temporary, then the real rows).  P holds the leading axes, H/R the real axis.
+int run_ndr(DevPlan &H, const RealTables &R, const std::vector<std::shared_ptr<DevPlan>> &P, const int *dims,
+            int inverse, const void *in, void *out, long long howmany, hipStream_t s)
+{
+    const int nlead = (int)P.size();
+    long long rows = 1;
+    for (int k = 0; k < nlead; ++k)
+        rows *= dims[k];
+    const long long nreal = dims[nlead], nbins = nreal / 2 + 1;
+    int rc = 0;
+    if (!inverse) {
+        rc = run_real_rows(H, R, 0, in, nreal / 2, out, nbins, rows * howmany, s);
+        long long outer = howmany, inner = rows * nbins;
+        for (int k = 0; k < nlead && rc == 0; ++k) {
+            inner /= dims[k];
+            rc = run_axis(*P[k], out, out, outer, inner, s);
+            outer *= dims[k];
+        }
+        return rc;
+    }
+    // the input is left untouched: the axes run out of place into a temporary
+    void *tmp = nullptr;
+    const size_t tb = cpx_bytes(H.type) * (size_t)(rows * nbins * howmany);
+    KFA_HIP(scratch_alloc(&tmp, tb, s));
+    long long outer = howmany, inner = rows * nbins;
+    const void *src = in;
+    for (int k = 0; k < nlead && rc == 0; ++k) {
+        inner /= dims[k];
+        rc = run_axis(*P[k], src, tmp, outer, inner, s);
+        outer *= dims[k];
+        src = tmp;
+    }
+    if (rc == 0 && nlead == 0)
+        rc = hipMemcpyAsync(tmp, in, tb, hipMemcpyDeviceToDevice, s) == hipSuccess ? 0 : fail(-3, "copy failed");
+    if (rc == 0)
+        rc = run_real_rows(H, R, 1, tmp, nbins, out, nreal / 2, rows * howmany, s);
+    (void)hipFreeAsync(tmp, s);
+    return rc;
+}
+
+}  // namespace kfa
+
+extern "C" {
+
 int kfa_nd(int ndims, const int *dims, const kfa_cplan *axes, const void *in, void *out,
            long long howmany, void *stream, int sync)
 {
@@ -1443,18 +1515,7 @@ int kfa_nd(int ndims, const int *dims, const kfa_cplan *axes, const void *in, vo
     if (rc == 0)
         rc = stage_out(out, bytes, s, O);
     if (rc == 0) {
-        // axis 0 first, as kiss_fftnd.c:181-186 does; fixed point rounds per axis, so the
-        // order is part of the result.  The first axis reads the input, later axes run in
-        // place on the output.
-        long long outer = howmany;
-        long long inner = total;
-        const void *src = I.dev;
-        for (int k = 0; k < ndims && rc == 0; ++k) {
-            inner /= dims[k];
-            rc = run_axis(*P[k], src, O.dev, outer, inner, s);
-            outer *= dims[k];
-            src = O.dev;
-        }
+        rc = run_nd(P, dims, I.dev, O.dev, howmany, s);
         if (rc == 0)
             rc = finish_out(O, s);
     }
@@ -1494,35 +1555,7 @@ int kfa_ndr(int ndims, const int *dims, const kfa_rplan *rpl, const kfa_cplan *a
     if (rc == 0)
         rc = stage_out(out, inverse ? real_bytes : cpx_bytes_, s, O);
     if (rc == 0) {
-        if (!inverse) {
-            // kiss_fftndr.c:99-109: real rows first, then the complex axes per bin column
-            rc = run_real_rows(*H, *R, 0, I.dev, nreal / 2, O.dev, nbins, rows * howmany, s);
-            long long outer = howmany, inner = rows * nbins;
-            for (int k = 0; k + 1 < ndims && rc == 0; ++k) {
-                inner /= dims[k];
-                rc = run_axis(*P[k], O.dev, O.dev, outer, inner, s);
-                outer *= dims[k];
-            }
-        } else {
-            // kiss_fftndr.c:121-131: complex axes per bin column first, then real rows.
-            // The input is left untouched: the axes run out of place into a temporary.
-            void *tmp = nullptr;
-            const size_t tb = cpx_bytes_;
-            KFA_HIP(scratch_alloc(&tmp, tb, s));
-            long long outer = howmany, inner = rows * nbins;
-            const void *src = I.dev;
-            for (int k = 0; k + 1 < ndims && rc == 0; ++k) {
-                inner /= dims[k];
-                rc = run_axis(*P[k], src, tmp, outer, inner, s);
-                outer *= dims[k];
-                src = tmp;
-            }
-            if (ndims == 1)
-                KFA_HIP(hipMemcpyAsync(tmp, I.dev, tb, hipMemcpyDeviceToDevice, s));
-            if (rc == 0)
-                rc = run_real_rows(*H, *R, 1, tmp, nbins, O.dev, nreal / 2, rows * howmany, s);
-            (void)hipFreeAsync(tmp, s);
-        }
+        rc = run_ndr(*H, *R, P, dims, inverse, I.dev, O.dev, howmany, s);
         if (rc == 0)
             rc = finish_out(O, s);
     }

@@ -62,6 +62,12 @@ int run_c2c(DevPlan &P, const void *in, long long istride, long long idist, void
             long long odist, long long howmany, hipStream_t s);
 int run_real_rows(DevPlan &H, const RealTables &R, int inverse, const void *in, long long in_hi, void *out,
                   long long out_hi, long long howmany, hipStream_t s);
+// N-D transforms of `howmany` arrays between device buffers (P: one plan per axis; for the
+// real transform P holds the leading axes and H / R the real last axis)
+int run_nd(const std::vector<std::shared_ptr<DevPlan>> &P, const int *dims, const void *in, void *out,
+           long long howmany, hipStream_t s);
+int run_ndr(DevPlan &H, const RealTables &R, const std::vector<std::shared_ptr<DevPlan>> &P, const int *dims,
+            int inverse, const void *in, void *out, long long howmany, hipStream_t s);
 
 // A caller buffer: used in place when the device can access it, else staged through a
 // stream-ordered device copy (stage_in copies host -> device; finish_out device -> host).

@@ -46,6 +46,9 @@ struct ServeReq {
     int n = 0;
     std::shared_ptr<DevPlan> P;     // complex plan (half length for real kinds)
     std::shared_ptr<RealTables> R;  // real kinds only
+    bool nd = false;                // N-D request: dims, one plan per (leading) axis in axes
+    std::vector<int> dims;
+    std::vector<std::shared_ptr<DevPlan>> axes;
     const void *in = nullptr;
     void *out = nullptr;
     long long howmany = 0;
@@ -155,7 +158,7 @@ int Server::launch_group(std::vector<std::shared_ptr<ServeReq>> &grp, hipStream_
     const bool real = r0.kind == KFA_SERVE_R2C || r0.kind == KFA_SERVE_C2R;
     const int fk = r0.kind == KFA_SERVE_R2C ? FK_R2C : r0.kind == KFA_SERVE_C2R ? FK_C2R : FK_C2C;
     const FastChoice *fc = nullptr;
-    int rc = P.single ? choose_fast(P, fk, s, &fc) : 0;
+    int rc = (P.single && !r0.nd) ? choose_fast(P, fk, s, &fc) : 0;
     if (rc)
         return rc;
     if (fc && fc->v) {
@@ -181,8 +184,18 @@ int Server::launch_group(std::vector<std::shared_ptr<ServeReq>> &grp, hipStream_
         ++g_launches;
         return fc->v->launch(a, s) ? fail(-3, "fast kernel launch failed") : 0;
     }
-    // no compiled kernel for this key (or a multi-pass length): one launch per request
+    // no compiled kernel for this key (or a multi-pass length, or N-D): one launch per request
     for (auto &r : grp) {
+        if (r->nd) {
+            if (r->kind == KFA_SERVE_R2C || r->kind == KFA_SERVE_C2R)
+                rc = run_ndr(*r->P, *r->R, r->axes, r->dims.data(), r->kind == KFA_SERVE_C2R, r->d_in, r->d_out,
+                             r->howmany, s);
+            else
+                rc = run_nd(r->axes, r->dims.data(), r->d_in, r->d_out, r->howmany, s);
+            if (rc)
+                return rc;
+            continue;
+        }
         switch (r->kind) {
         case KFA_SERVE_C2C_FWD:
         case KFA_SERVE_C2C_INV:
@@ -238,8 +251,9 @@ void Server::run()
         lk.unlock();
         // group by (kind, length), keeping arrival order inside a group
         std::map<std::pair<int, int>, std::vector<std::shared_ptr<ServeReq>>> groups;
-        for (auto &r : ready)
-            groups[{r->kind, r->n}].push_back(r);
+        for (auto &r : ready)  // N-D requests run alone (a key of their own)
+            groups[r->nd ? std::make_pair(16 + r->kind, (int)(r->id & 0x7fffffff)) : std::make_pair(r->kind, r->n)]
+                .push_back(r);
         for (auto &kv : groups) {
             auto &all = kv.second;
             for (size_t off = 0; off < all.size(); off += (size_t)max_group) {
@@ -386,6 +400,79 @@ long long kfa_server_submit(void *srv, int kind, const kfa_cplan *cp, const kfa_
         const size_t one = cpx_bytes(S->type) * (size_t)r->P->n * (size_t)howmany;
         r->need += (real ? 2 : 1) * std::min(one, kScratchChunk);
     }
+    r->t_submit = now_s();
+    {
+        std::lock_guard<std::mutex> g(S->mu);
+        r->id = S->next_id++;
+        S->live[r->id] = r;
+        S->pending.push_back(r);
+        S->submitted++;
+        if (S->t_first == 0)
+            S->t_first = r->t_submit;
+    }
+    S->cv_work.notify_all();
+    return r->id;
+}
+
+long long kfa_server_submit_nd(void *srv, int kind, int ndims, const int *dims, const kfa_cplan *axes,
+                               const kfa_rplan *rp, const void *in, void *out, long long howmany)
+{
+    Server *S = static_cast<Server *>(srv);
+    if (!S || !in || !out || howmany <= 0 || ndims <= 0 || !dims || kind < 0 || kind > 3)
+        return fail(-2, "server submit: bad arguments");
+    const bool real = kind == KFA_SERVE_R2C || kind == KFA_SERVE_C2R;
+    auto r = std::make_shared<ServeReq>();
+    r->nd = true;
+    r->kind = kind;
+    r->howmany = howmany;
+    r->in = in;
+    r->out = out;
+    r->dims.assign(dims, dims + ndims);
+    long long total = 1;
+    for (int k = 0; k < ndims; ++k) {
+        if (dims[k] <= 0)
+            return fail(-2, "server submit: bad dimension");
+        total *= dims[k];
+    }
+    r->n = (int)std::min<long long>(total, 0x7fffffff);
+    int rc;
+    bool multipass = false;
+    const int nplans = real ? ndims - 1 : ndims;
+    r->axes.resize(nplans);
+    for (int k = 0; k < nplans; ++k) {
+        if ((rc = build_cplan(&axes[k], S->dev, r->axes[k])))
+            return rc;
+        multipass |= !r->axes[k]->single;
+    }
+    if (real) {
+        if (!rp || (dims[ndims - 1] & 1))
+            return fail(-2, "server submit: real N-D requests need an even last dimension");
+        if ((rc = build_cplan(&rp->half, S->dev, r->P)) || (rc = build_rtab(rp, S->dev, r->R)))
+            return rc;
+        multipass |= !r->P->single;
+    } else {
+        r->P = r->axes[0];
+    }
+    const size_t cb = cpx_bytes(S->type), sb = cb / 2;
+    const long long last = dims[ndims - 1];
+    const size_t cpx_all = cb * (size_t)(total / last * (last / 2 + 1)) * (size_t)howmany;
+    const size_t real_all = sb * (size_t)total * (size_t)howmany;
+    if (!real)
+        r->in_bytes = r->out_bytes = cb * (size_t)total * (size_t)howmany;
+    else if (kind == KFA_SERVE_R2C) {
+        r->in_bytes = real_all;
+        r->out_bytes = cpx_all;
+    } else {
+        r->in_bytes = cpx_all;
+        r->out_bytes = real_all;
+    }
+    r->stage_in = !device_accessible(in);
+    r->stage_out = !device_accessible(out);
+    r->need = (r->stage_in ? r->in_bytes : 0) + (r->stage_out ? r->out_bytes : 0);
+    if (kind == KFA_SERVE_C2R)  // the inverse runs its complex axes into a temporary
+        r->need += cpx_all;
+    if (multipass)
+        r->need += 2 * kScratchChunk;
     r->t_submit = now_s();
     {
         std::lock_guard<std::mutex> g(S->mu);

@@ -6,6 +6,7 @@
  * kept for the server's lifetime.
  */
 #include <pthread.h>
+#include <string.h>
 
 #include "kf_internal.h"
 #include "kiss_fft_serve.h"
@@ -13,6 +14,8 @@
 struct kf_plan_entry {
     int kind;
     int nfft;
+    int ndims;      /* 0: a 1-D plan of nfft points; else an N-D plan of dims */
+    int dims[32];
     void *cfg;
     struct kf_plan_entry *next;
 };
@@ -44,7 +47,7 @@ static void *plan_for(kiss_fft_server s, int kind, int nfft)
     void *cfg = NULL;
     pthread_mutex_lock(&s->mu);
     for (e = s->plans; e; e = e->next)
-        if (e->kind == kind && e->nfft == nfft)
+        if (e->kind == kind && e->nfft == nfft && e->ndims == 0)
             break;
     if (!e) {
         if (kind == KISS_FFT_SERVE_R2C || kind == KISS_FFT_SERVE_C2R)
@@ -56,6 +59,7 @@ static void *plan_for(kiss_fft_server s, int kind, int nfft)
             if (e) {
                 e->kind = kind;
                 e->nfft = nfft;
+                e->ndims = 0;
                 e->cfg = cfg;
                 e->next = s->plans;
                 s->plans = e;
@@ -86,6 +90,65 @@ long long kiss_fft_server_submit(kiss_fft_server s, int kind, int nfft, const vo
     kfa_cplan cp;
     kf_cplan_of((kiss_fft_cfg)cfg, &cp);
     return kfa_server_submit(s->engine, kind, &cp, NULL, nfft, in, out, howmany);
+}
+
+static void *plan_for_nd(kiss_fft_server s, int kind, int ndims, const int *dims)
+{
+    struct kf_plan_entry *e;
+    void *cfg = NULL;
+    pthread_mutex_lock(&s->mu);
+    for (e = s->plans; e; e = e->next)
+        if (e->kind == kind && e->ndims == ndims && memcmp(e->dims, dims, sizeof(int) * (size_t)ndims) == 0)
+            break;
+    if (!e) {
+        if (kind == KISS_FFT_SERVE_R2C || kind == KISS_FFT_SERVE_C2R)
+            cfg = kiss_fftndr_alloc(dims, ndims, kind == KISS_FFT_SERVE_C2R, NULL, NULL);
+        else
+            cfg = kiss_fftnd_alloc(dims, ndims, kind == KISS_FFT_SERVE_C2C_INV, NULL, NULL);
+        if (cfg) {
+            e = (struct kf_plan_entry *)calloc(1, sizeof(*e));
+            if (e) {
+                e->kind = kind;
+                e->ndims = ndims;
+                memcpy(e->dims, dims, sizeof(int) * (size_t)ndims);
+                e->cfg = cfg;
+                e->next = s->plans;
+                s->plans = e;
+            } else {
+                free(cfg);
+                cfg = NULL;
+            }
+        }
+    }
+    if (e)
+        cfg = e->cfg;
+    pthread_mutex_unlock(&s->mu);
+    return cfg;
+}
+
+long long kiss_fft_server_submit_nd(kiss_fft_server s, int kind, int ndims, const int *dims, const void *in, void *out,
+                                    long long howmany)
+{
+    kfa_cplan axes[32];
+    kfa_rplan rp;
+    int k;
+    if (!s || kind < 0 || kind > 3 || ndims < 1 || ndims > 32 || !dims)
+        return -2;
+    void *cfg = plan_for_nd(s, kind, ndims, dims);
+    if (!cfg)
+        return -2;
+    if (kind == KISS_FFT_SERVE_R2C || kind == KISS_FFT_SERVE_C2R) {
+        kiss_fftndr_cfg st = (kiss_fftndr_cfg)cfg;
+        const int nlead = st->cfg_nd ? st->cfg_nd->ndims : 0;
+        for (k = 0; k < nlead; ++k)
+            kf_cplan_of(st->cfg_nd->states[k], &axes[k]);
+        kf_rplan_of(st->cfg_r, &rp);
+        return kfa_server_submit_nd(s->engine, kind, ndims, dims, axes, &rp, in, out, howmany);
+    }
+    kiss_fftnd_cfg st = (kiss_fftnd_cfg)cfg;
+    for (k = 0; k < ndims; ++k)
+        kf_cplan_of(st->states[k], &axes[k]);
+    return kfa_server_submit_nd(s->engine, kind, ndims, dims, axes, NULL, in, out, howmany);
 }
 
 int kiss_fft_server_wait(kiss_fft_server s, long long ticket, double *latency_s)

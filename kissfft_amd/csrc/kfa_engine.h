@@ -84,6 +84,10 @@ KFA_HIDDEN void *kfa_server_create(int type, long long budget_bytes, int nstream
  * real kinds).  Returns a ticket > 0 or a negative code. */
 KFA_HIDDEN long long kfa_server_submit(void *srv, int kind, const kfa_cplan *cp, const kfa_rplan *rp, int nfft,
                                        const void *in, void *out, long long howmany);
+/* N-D request: complex kinds take one plan per axis in `axes`; real kinds take the real
+ * plan `rp` of the last axis and the leading axes' plans in `axes`. */
+KFA_HIDDEN long long kfa_server_submit_nd(void *srv, int kind, int ndims, const int *dims, const kfa_cplan *axes,
+                                          const kfa_rplan *rp, const void *in, void *out, long long howmany);
 KFA_HIDDEN int kfa_server_wait(void *srv, long long ticket, double *latency_s);
 KFA_HIDDEN int kfa_server_drain(void *srv);
 KFA_HIDDEN int kfa_server_get_stats(void *srv, kfa_server_stats *st);

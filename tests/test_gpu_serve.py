@@ -121,3 +121,64 @@ def test_q15_served_bit_exact():
         srv.wait(t)
         np.testing.assert_array_equal(y, O.fft(x))
     srv.close()
+
+
+def test_nd_requests_in_the_mix():
+    import torch
+    lib = kissfft_amd.load("float")
+    srv = lib.server(0, 3, 16)
+    rng = np.random.default_rng(64)
+    jobs = []
+    # N-D requests (complex 3-D, real 2-D forward and inverse) between 1-D ones
+    for i in range(4):
+        x = rng.uniform(-1, 1, (5, 16, 12, 10, 2)).astype(np.float32)
+        y = np.zeros_like(x)
+        jobs.append(("nd", (16, 12, 10), x, y, srv.submit_nd(C2C_FWD, (16, 12, 10), x, y, 5)))
+        xr = torch.from_numpy(rng.standard_normal((3, 64, 50)).astype(np.float32)).cuda()
+        Xr = torch.empty(3, 64, 26, 2, device="cuda")
+        jobs.append(("ndr", (64, 50), xr, Xr, srv.submit_nd(R2C, (64, 50), xr, Xr, 3)))
+        X = rng.uniform(-1, 1, (2, 32, 9, 2)).astype(np.float32)
+        yr = np.zeros((2, 32, 16), np.float32)
+        jobs.append(("ndri", (32, 16), X, yr, srv.submit_nd(C2R, (32, 16), X, yr, 2)))
+        x1 = rng.uniform(-1, 1, (40, 1200, 2)).astype(np.float32)
+        y1 = np.zeros_like(x1)
+        jobs.append(("c2c", 1200, x1, y1, srv.submit(C2C_FWD, 1200, x1, y1, 40)))
+    for kind, dims, x, y, t in jobs:
+        srv.wait(t)
+    for kind, dims, x, y, t in jobs:
+        say("served", kind, dims)
+        if kind == "nd":
+            want = np.zeros_like(x)
+            lib.fftnd_batch(lib.fftnd_alloc(dims), x, want, x.shape[0])
+            np.testing.assert_array_equal(y, want)
+            assert rel_l2(cplx(y), np.fft.fftn(cplx(x), axes=(1, 2, 3))) <= ABS_FLOAT_BOUND
+        elif kind == "ndr":
+            want = torch.empty_like(y)
+            lib.fftndr_batch(lib.fftndr_alloc(dims), x, want, x.shape[0])
+            torch.cuda.synchronize()
+            assert torch.equal(y, want)
+        elif kind == "ndri":
+            want = np.zeros_like(y)
+            lib.fftndri_batch(lib.fftndr_alloc(dims, True), x, want, x.shape[0])
+            np.testing.assert_array_equal(y, want)
+        else:
+            np.testing.assert_array_equal(y, alone(lib, C2C_FWD, dims, x))
+    st = srv.stats()
+    assert st["completed"] == len(jobs) and st["failed"] == 0 and st["in_use_bytes"] == 0
+    srv.close()
+
+
+def test_q15_nd_served_bit_exact():
+    lib = kissfft_amd.load("int16_t")
+    O = Oracle("int16_t")
+    srv = lib.server(0, 2, 8)
+    rng = np.random.default_rng(65)
+    x = rng.integers(-32768, 32768, (3, 12, 60, 2)).astype(np.int16)
+    y = np.zeros_like(x)
+    srv.wait(srv.submit_nd(C2C_FWD, (12, 60), x, y, 3))
+    np.testing.assert_array_equal(y, O.fftnd(x, (12, 60)))
+    xr = rng.integers(-32768, 32768, (2, 8, 40)).astype(np.int16)
+    Y = np.zeros((2, 8, 21, 2), np.int16)
+    srv.wait(srv.submit_nd(R2C, (8, 40), xr, Y, 2))
+    np.testing.assert_array_equal(Y, O.fftndr(xr, (8, 40)))
+    srv.close()
