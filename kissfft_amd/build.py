@@ -19,6 +19,7 @@ instructions disabled.
 """
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -77,12 +78,14 @@ def build(datatypes=None, verbose: bool = True) -> list[str]:
     datatypes = list(datatypes or DATATYPES)
     objdir = os.path.join(LIBDIR, "obj")
     os.makedirs(objdir, exist_ok=True)
-    engine_objs = []
-    for src in ENGINE_SOURCES:
-        obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
-        _run([HIPCC, *HIP_FLAGS, "-I", CSRC, "-I", INCLUDE, "-c", os.path.join(CSRC, "engine", src), "-o", obj],
-             verbose)
-        engine_objs.append(obj)
+    engine_objs = [os.path.join(objdir, os.path.splitext(src)[0] + ".o") for src in ENGINE_SOURCES]
+    cmds = [[HIPCC, *HIP_FLAGS, "-I", CSRC, "-I", INCLUDE, "-c", os.path.join(CSRC, "engine", src), "-o", obj]
+            for src, obj in zip(ENGINE_SOURCES, engine_objs)]
+    # the HIP translation units are independent: compile them side by side
+    workers = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "0") or 0) or (os.cpu_count() or 1)))
+    with concurrent.futures.ThreadPoolExecutor(max_workers=workers) as pool:
+        for f in [pool.submit(_run, c, verbose) for c in cmds]:
+            f.result()
     built = []
     for dt in datatypes:
         host_objs = []
