@@ -364,6 +364,35 @@ static TileCache &tile_cache()
 }
 
 // The tile variant for (n, inverse, mode) on the current device, or v == nullptr.
+// Tuning input: random-looking data of the scalar kind (float uniform in [-1, 1); Q15 /
+// Q31 halves uniform over half their range).  The fixed-point kernels are arithmetic
+// bound, and their clock -- hence the ranking of variants -- depends on the data.
+__global__ void k_fill_tune(unsigned *p, size_t nwords, int type)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += (size_t)gridDim.x * blockDim.x) {
+        unsigned z = (unsigned)i * 0x9E3779B9u + 0x7F4A7C15u;
+        z = (z ^ (z >> 16)) * 0x85EBCA6Bu;
+        z = (z ^ (z >> 13)) * 0xC2B2AE35u;
+        z ^= z >> 16;
+        unsigned w;
+        if (type == KFA_F32)
+            w = __float_as_uint((float)(z >> 8) * (1.0f / 8388608.0f) - 1.0f);
+        else if (type == KFA_Q15)
+            w = (((z & 0x7fffu) - 0x4000u) & 0xffffu) | ((((z >> 16) & 0x7fffu) - 0x4000u) << 16);
+        else
+            w = (unsigned)((int)z >> 1);
+        p[i] = w;
+    }
+}
+
+static int fill_tune(void *p, size_t bytes, int type, hipStream_t s)
+{
+    const size_t nw = bytes / 4;
+    hipLaunchKernelGGL(k_fill_tune, dim3(2048), dim3(256), 0, s, (unsigned *)p, nw, type);
+    KFA_HIP(hipGetLastError());
+    return 0;
+}
+
 static int launch_tile(const TileChoice &tc, TileArgs a, hipStream_t s);
 
 // Times the tile candidates of one (length, direction, mapping) on a scratch layout of
@@ -380,7 +409,8 @@ static int tune_tiles(int n, int mode, std::vector<TileChoice> &cand, hipStream_
     KFA_HIP(scratch_alloc(&bout, bytes, s));
     KFA_HIP(scratch_alloc(&whi, hib, s));
     KFA_HIP(scratch_alloc(&wlo, 8 * 1024, s));
-    KFA_HIP(hipMemsetAsync(bin, 0x3c, bytes, s));
+    if (int rc0 = fill_tune(bin, bytes, KFA_F32, s))
+        return rc0;
     KFA_HIP(hipMemsetAsync(whi, 0x3c, hib, s));
     KFA_HIP(hipMemsetAsync(wlo, 0x3c, 8 * 1024, s));
     TileArgs a{};
@@ -771,8 +801,9 @@ int choose_fast(DevPlan &P, int kind, hipStream_t s, const FastChoice **out, lon
         KFA_HIP(scratch_alloc(&bin, bytes, s));
         KFA_HIP(scratch_alloc(&bout, bytes, s));
         KFA_HIP(scratch_alloc(&brtw, cb * (size_t)(P.n / 2 + 1), s));
-        // a non-zero constant pattern (float 0.0115, int16 15420): representative power draw
-        KFA_HIP(hipMemsetAsync(bin, 0x3c, bytes, s));
+        // random-looking data of the plan's kind: representative power draw and clock
+        if (int rc0 = fill_tune(bin, bytes, P.type, s))
+            return rc0;
         KFA_HIP(hipMemsetAsync(brtw, 0x3c, cb * (size_t)(P.n / 2 + 1), s));
         hipEvent_t e0, e1;
         KFA_HIP(hipEventCreate(&e0));

@@ -150,10 +150,12 @@ const std::vector<FastVariant> &fast_registry()
         add_variant<F32, FK_C2C, 192, 1, false, false, 3, 16, 8, 8>(*v, KFA_F32, "c2c3072_31688_t192_sb");
         add_variant<F32, FK_C2C, 256, 1, false, false, 16, 16, 16>(*v, KFA_F32, "c2c4096_161616_t256_sb");
         // ---- Q15 (reference schedules), packed-word butterflies ----
-        add_variant<Q15P, FK_C2C, 64, 4, true, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c512_t64x4_db");
-        add_variant<Q15P, FK_C2C, 64, 4, false, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c512_t64x4_sb");
-        add_variant<Q15P, FK_C2C, 128, 2, false, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c512_t128x2_sb");
         add_variant<Q15P, FK_C2C, 32, 8, false, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c512_t32x8_sb");
+        add_variant<Q15P, FK_C2C, 16, 16, false, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c512_t16x16_sb");
+        add_variant<Q15P, FK_C2C, 32, 8, true, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c512_t32x8_db");
+        add_variant<Q15P, FK_C2C, 64, 4, false, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c512_t64x4_sb");
+        add_variant<Q15P, FK_C2C, 64, 4, true, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c512_t64x4_db");
+        add_variant<Q15P, FK_C2C, 128, 2, false, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c512_t128x2_sb");
         add_variant<Q15P, FK_C2C, 240, 1, false, false, 5, 5, 3, 4, 4>(*v, KFA_Q15, "q15p_c2c1200_t240_sb");
         add_variant<Q15P, FK_C2C, 64, 4, false, false, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c256_t64x4_sb");
         add_variant<Q15P, FK_C2C, 128, 2, false, false, 4, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c1024_t128x2_sb");
