@@ -631,28 +631,35 @@ __global__ void __launch_bounds__(T *LPB) k_fast(FastArgs a)
     const S *rtw = reinterpret_cast<const S *>(a.rtw);
 
     if constexpr (KIND == FK_C2R && !FUSE) {
-        // Hermitian merge into bufA (kiss_fftr.c:131-153); bins read from HBM
+        // Hermitian merge into bufA (kiss_fftr.c:131-153); bins read from HBM.  Every load
+        // is issued first (indices clamped into the row), then the pairs are combined:
+        // loads behind the loop's exit test would wait one iteration at a time.
         constexpr int NK = N / 2 + 1;
         constexpr int JK = (NK + T - 1) / T;
+        C xa[JK], bz[JK], tw[JK];
+#pragma unroll
+        for (int j = 0; j < JK; ++j) {
+            const int k = t + j * T;
+            const int kc = k < NK ? k : NK - 1;
+            xa[j] = valid ? O::load(hbm_ld(gin + (long long)kc * a.in_el)) : O::zero();
+            bz[j] = valid ? O::load(hbm_ld(gin + (long long)(N - kc) * a.in_el)) : O::zero();
+            tw[j] = O::load(rtw[kc > 0 ? kc - 1 : 0]);
+        }
 #pragma unroll
         for (int j = 0; j < JK; ++j) {
             const int k = t + j * T;
             if (k >= NK)
                 break;
-            if (k == 0) {
-                C x0 = valid ? O::load(hbm_ld(gin)) : O::zero();
-                C xn = valid ? O::load(hbm_ld(gin + (long long)N * a.in_el)) : O::zero();
-                C d{O::add(x0.r, xn.r), O::sub(x0.r, xn.r)};
+            if (k == 0) {  // DC and Nyquist: xa = X[0], bz = X[N]
+                C d{O::add(xa[j].r, bz[j].r), O::sub(xa[j].r, bz[j].r)};
                 bufA[0] = O::pack(O::fixdiv(d, 2));
                 continue;
             }
-            C xa = valid ? O::load(hbm_ld(gin + (long long)k * a.in_el)) : O::zero();
-            C bz = valid ? O::load(hbm_ld(gin + (long long)(N - k) * a.in_el)) : O::zero();
-            C xb{bz.r, O::neg(bz.i)};
-            xa = O::fixdiv(xa, 2);
+            C xb{bz[j].r, O::neg(bz[j].i)};
+            const C x1 = O::fixdiv(xa[j], 2);
             xb = O::fixdiv(xb, 2);
-            C e = O::cadd(xa, xb), d = O::csub(xa, xb);
-            C o = O::cmul(d, O::load(rtw[k - 1]));
+            C e = O::cadd(x1, xb), d = O::csub(x1, xb);
+            C o = O::cmul(d, tw[j]);
             C lo = O::csub(e, o);
             lo.i = O::neg(lo.i);
             if (2 * k != N)
@@ -672,27 +679,34 @@ __global__ void __launch_bounds__(T *LPB) k_fast(FastArgs a)
         constexpr int JK = (NK + T - 1) / T;
         if (!valid)
             return;
+        // operands first (LDS pairs and split twiddles, indices clamped), then the stores
+        C za[JK], zb[JK], tw[JK];
+#pragma unroll
+        for (int j = 0; j < JK; ++j) {
+            const int k = t + j * T;
+            const int kc = k < NK ? k : NK - 1;
+            za[j] = O::load(z[kc]);
+            zb[j] = O::load(z[kc == 0 ? 0 : N - kc]);
+            tw[j] = O::load(rtw[kc > 0 ? kc - 1 : 0]);
+        }
 #pragma unroll
         for (int j = 0; j < JK; ++j) {
             const int k = t + j * T;
             if (k >= NK)
                 break;
             if (k == 0) {
-                C dc = O::fixdiv(O::load(z[0]), 2);
+                C dc = O::fixdiv(za[j], 2);
                 hbm_st(gout, O::pack(C{O::add(dc.r, dc.i), O::zero().r}));
                 hbm_st(gout + (long long)N * a.out_el, O::pack(C{O::sub(dc.r, dc.i), O::zero().r}));
                 continue;
             }
-            C xa = O::load(z[k]);
-            C bz = O::load(z[N - k]);
-            C xb{bz.r, O::neg(bz.i)};
-            xa = O::fixdiv(xa, 2);
-            xb = O::fixdiv(xb, 2);
+            C xa = O::fixdiv(za[j], 2);
+            C xb = O::fixdiv(C{zb[j].r, O::neg(zb[j].i)}, 2);
             C f1 = O::cadd(xa, xb), f2 = O::csub(xa, xb);
-            C tw = O::cmul(f2, O::load(rtw[k - 1]));
+            C w = O::cmul(f2, tw[j]);
             if (2 * k != N)
-                hbm_st(gout + (long long)k * a.out_el, O::pack(C{O::half_add(f1.r, tw.r), O::half_add(f1.i, tw.i)}));
-            hbm_st(gout + (long long)(N - k) * a.out_el, O::pack(C{O::half_sub(f1.r, tw.r), O::half_sub(tw.i, f1.i)}));
+                hbm_st(gout + (long long)k * a.out_el, O::pack(C{O::half_add(f1.r, w.r), O::half_add(f1.i, w.i)}));
+            hbm_st(gout + (long long)(N - k) * a.out_el, O::pack(C{O::half_sub(f1.r, w.r), O::half_sub(w.i, f1.i)}));
         }
     }
 }
