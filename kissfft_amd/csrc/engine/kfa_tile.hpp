@@ -62,7 +62,8 @@ __device__ __forceinline__ void tile_stages(std::integer_sequence<int, S...>, ty
 // fetched into that XCD's L2 once.
 __device__ __forceinline__ long long tile_of_block(long long b, long long ntiles, int G)
 {
-    if (G <= 1 || ntiles % (8LL * G) != 0)
+    const long long full = ntiles / (8LL * G) * (8LL * G);  // complete groups; the tail in order
+    if (G <= 1 || b >= full)
         return b;
     const long long x = b % 8, s = b / 8;
     return ((s / G) * 8 + x) * G + s % G;
