@@ -34,6 +34,7 @@
 #include "kfa_lines.hpp"
 #include "kfa_blue.hpp"
 #include "kfa_tile.hpp"
+#include "kfa_tile_real.hpp"
 
 namespace kfa {
 
@@ -355,6 +356,7 @@ struct TileCache {
     std::mutex mu;
     std::map<std::tuple<int, int, int, int>, TileChoice> choice;            // dev, n, inv, mode
     std::map<std::tuple<int, long long, int>, std::pair<void *, void *>> fs; // dev, N, inv -> hi, lo
+    std::map<std::tuple<int, long long, int, int>, std::pair<void *, void *>> rs;  // dev, nc, n, inv
 };
 
 static TileCache &tile_cache()
@@ -394,25 +396,31 @@ static int fill_tune(void *p, size_t bytes, int type, hipStream_t s)
 }
 
 static int launch_tile(const TileChoice &tc, TileArgs a, hipStream_t s);
+static int launch_rtile(const TileChoice &tc, RealTileArgs a, long long howmany, hipStream_t s);
 
 // Times the tile candidates of one (length, direction, mapping) on a scratch layout of
-// that mapping (1024 lines per group, about 256 MiB moved per launch) and returns the
-// index of the fastest; interleaved rounds as for the line kernels.
+// that mapping (1024 lines per group, about 128 MiB read per launch) and returns the
+// index of the fastest; interleaved rounds as for the line kernels.  The fused real
+// mappings time transforms of n * 1024 complex values.
 static int tune_tiles(int n, int mode, std::vector<TileChoice> &cand, hipStream_t s, size_t *best)
 {
     const long long C = 1024;
     const long long outer = std::max(1LL, (16LL << 20) / ((long long)n * C));
-    const size_t bytes = 8 * (size_t)n * (size_t)C * (size_t)outer;
-    void *bin = nullptr, *bout = nullptr, *whi = nullptr, *wlo = nullptr;
+    const bool real = mode == TILE_RSPLIT || mode == TILE_RMERGE;
+    const size_t bytes = 8 * (size_t)(n * C + 1) * (size_t)outer;
+    void *bin = nullptr, *bout = nullptr, *whi = nullptr, *wlo = nullptr, *rtw = nullptr;
     const size_t hib = 8 * (size_t)((long long)n * C / 1024 + 1);
+    const size_t rtb = 8 * (size_t)(n + C);
     KFA_HIP(scratch_alloc(&bin, bytes, s));
     KFA_HIP(scratch_alloc(&bout, bytes, s));
     KFA_HIP(scratch_alloc(&whi, hib, s));
     KFA_HIP(scratch_alloc(&wlo, 8 * 1024, s));
+    KFA_HIP(scratch_alloc(&rtw, rtb, s));
     if (int rc0 = fill_tune(bin, bytes, KFA_F32, s))
         return rc0;
     KFA_HIP(hipMemsetAsync(whi, 0x3c, hib, s));
     KFA_HIP(hipMemsetAsync(wlo, 0x3c, 8 * 1024, s));
+    KFA_HIP(hipMemsetAsync(rtw, 0x3c, rtb, s));
     TileArgs a{};
     a.in = bin;
     a.out = bout;
@@ -425,18 +433,29 @@ static int tune_tiles(int n, int mode, std::vector<TileChoice> &cand, hipStream_
     a.out_el = C;
     a.in_lo = mode == TILE_ROWS_TO_COLS ? n : 1;
     a.in_el = mode == TILE_ROWS_TO_COLS ? 1 : C;
+    RealTileArgs ra{};
+    ra.in = bin;
+    ra.out = bout;
+    ra.rtw_a = rtw;
+    ra.rtw_b = (const char *)rtw + 8 * C;
+    ra.tw_hi = whi;
+    ra.tw_lo = wlo;
+    ra.nc = (long long)n * C;
+    ra.in_hi = ra.out_hi = ra.nc + 1;
+    ra.other = (int)C;
+    auto run = [&](size_t i) { return real ? launch_rtile(cand[i], ra, outer, s) : launch_tile(cand[i], a, s); };
     hipEvent_t e0, e1;
     KFA_HIP(hipEventCreate(&e0));
     KFA_HIP(hipEventCreate(&e1));
     std::vector<float> t(cand.size(), 1e30f);
     int rc = 0;
     for (size_t i = 0; i < cand.size() && rc == 0; ++i)
-        rc = launch_tile(cand[i], a, s);  // warm-up
+        rc = run(i);  // warm-up
     for (int round = 0; round < 3 && rc == 0; ++round)
         for (size_t i = 0; i < cand.size() && rc == 0; ++i) {
             KFA_HIP(hipEventRecord(e0, s));
             for (int r = 0; r < 3 && rc == 0; ++r)
-                rc = launch_tile(cand[i], a, s);
+                rc = run(i);
             KFA_HIP(hipEventRecord(e1, s));
             KFA_HIP(hipEventSynchronize(e1));
             float ms = 0;
@@ -452,7 +471,7 @@ static int tune_tiles(int n, int mode, std::vector<TileChoice> &cand, hipStream_
             fprintf(stderr, "[kissfft_amd]   %-32s mode %d %8.1f us\n", cand[i].v->name, mode, 1e3f * t[i] / 3);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    for (void *p : {bin, bout, whi, wlo})
+    for (void *p : {bin, bout, whi, wlo, rtw})
         (void)hipFreeAsync(p, s);
     KFA_HIP(hipStreamSynchronize(s));
     return rc;
@@ -536,6 +555,46 @@ static int fourstep_twiddles(long long N, int inverse, const void **hi, const vo
     return 0;
 }
 
+// The fused real passes' split / merge twiddles for half length nc, line length n and
+// other = nc / n lines: a[i] = exp(-+i*pi*(i/nc + 1/2)), i < other, and
+// b[e] = exp(-+i*pi*e/n), e < n (kiss_fftr.c:48-53 restated as a product).
+static int real_split_twiddles(long long nc, int n, int inverse, const void **ta, const void **tb)
+{
+    int dev;
+    int rc = current_device(&dev);
+    if (rc)
+        return rc;
+    auto &C = tile_cache();
+    std::lock_guard<std::mutex> g(C.mu);
+    const auto key = std::make_tuple(dev, nc, n, inverse);
+    auto it = C.rs.find(key);
+    if (it == C.rs.end()) {
+        const double pi = 3.14159265358979323846264338327950288;
+        const double sg = inverse ? 1.0 : -1.0;
+        const long long other = nc / n;
+        std::vector<float> ha(2 * other), hb(2 * (size_t)n);
+        for (long long i = 0; i < other; ++i) {
+            const double ph = sg * pi * ((double)i / (double)nc + 0.5);
+            ha[2 * i] = (float)cos(ph);
+            ha[2 * i + 1] = (float)sin(ph);
+        }
+        for (int e = 0; e < n; ++e) {
+            const double ph = sg * pi * (double)e / (double)n;
+            hb[2 * e] = (float)cos(ph);
+            hb[2 * e + 1] = (float)sin(ph);
+        }
+        void *da = nullptr, *db = nullptr;
+        KFA_HIP(hipMalloc(&da, ha.size() * 4));
+        KFA_HIP(hipMalloc(&db, hb.size() * 4));
+        KFA_HIP(hipMemcpy(da, ha.data(), ha.size() * 4, hipMemcpyHostToDevice));
+        KFA_HIP(hipMemcpy(db, hb.data(), hb.size() * 4, hipMemcpyHostToDevice));
+        it = C.rs.emplace(key, std::make_pair(da, db)).first;
+    }
+    *ta = it->second.first;
+    *tb = it->second.second;
+    return 0;
+}
+
 static int launch_tile(const TileChoice &tc, TileArgs a, hipStream_t s)
 {
     a.stw = tc.d_stw;
@@ -547,6 +606,24 @@ static int launch_tile(const TileChoice &tc, TileArgs a, hipStream_t s)
         return fail(-2, "too many lines (%lld) for one launch", a.nlines);
     ++g_launches;
     return tc.v->launch(a, tiles, s) ? fail(-3, "tile kernel launch failed") : 0;
+}
+
+// A fused real pass over `howmany` transforms: a.other / 2 line pairs each, TL / 2 per
+// workgroup.
+static int launch_rtile(const TileChoice &tc, RealTileArgs a, long long howmany, hipStream_t s)
+{
+    a.stw = tc.d_stw;
+    const int np = tc.v->TL / 2;
+    if (a.other < 32 || a.other % 2 || (a.other / 2) % np)
+        return fail(-2, "fused real pass: %d lines do not tile by %d pairs", a.other, np);
+    const long long blocks = howmany * (a.other / 2 / np);
+    if (blocks > 0x7fffffffLL)
+        return fail(-2, "too many transforms (%lld) for one launch", howmany);
+    a.tiles = blocks;
+    // neighbouring tiles share the cache lines of their bin runs (np values each)
+    a.xcd_group = np * 8 < 128 ? 128 / (np * 8) : 1;
+    ++g_launches;
+    return tc.v->launch_real(a, blocks, s) ? fail(-3, "fused real tile kernel launch failed") : 0;
 }
 
 // ------------------------------------------------------------------------------------
@@ -925,7 +1002,7 @@ int stage_in(const void *p, size_t bytes, hipStream_t s, Operand &op)
     return 0;
 }
 
-int stage_out(void *p, size_t bytes, hipStream_t s, Operand &op)
+int stage_out(void *p, size_t bytes, hipStream_t s, Operand &op, bool gaps)
 {
     op.bytes = bytes;
     if (device_accessible(p)) {
@@ -935,6 +1012,10 @@ int stage_out(void *p, size_t bytes, hipStream_t s, Operand &op)
     op.staged = true;
     op.host = p;
     KFA_HIP(scratch_alloc(&op.dev, bytes ? bytes : 1, s));
+    if (gaps && bytes) {  // the copy back covers the whole span
+        KFA_HIP(hipMemcpyAsync(op.dev, p, bytes, hipMemcpyHostToDevice, s));
+        g_h2d += (long long)bytes;
+    }
     return 0;
 }
 
@@ -1165,9 +1246,111 @@ static int launch_real_pass(int merge, const void *src, void *dst, const void *s
     return 0;
 }
 
-// Real rows whose half length needs the multi-pass complex transform: the complex pass
-// runs into a stream-ordered temporary (nc values per line) and the split (forward) or
-// merge (inverse) runs as its own HBM pass on that temporary.
+// Float real rows of half length nc = N1*N2 in two HBM passes, the split / merge fused
+// into the four-step passes (kfa_tile_real.hpp).  Forward: the column pass of z (length
+// N2, four-step twiddle) into tmp, then row pairs of tmp (length N1) split into the bins.
+// Inverse: column pairs of the bins merged and transformed (length N2, twiddle) into tmp,
+// then the rows of tmp (length N1) written as columns of z.  tmp holds (nc + 16) * howmany
+// + 16 values.  Returns 1 (nothing done) when no registered split of nc fits.
+static int run_real_fourstep(DevPlan &H, const RealTables &R, int inverse, const void *in, long long in_hi,
+                             void *out, long long out_hi, long long howmany, void *tmp, hipStream_t s)
+{
+    if (H.type != KFA_F32 || !env_flag("KISSFFT_AMD_FAST", true) || !env_flag("KISSFFT_AMD_REAL_FUSED", true))
+        return 1;
+    const long long nc = H.n;
+    // the fused mapping sets the length of its own half: N1 (split rows) or N2 (merge columns)
+    const int fused_mode = inverse ? TILE_RMERGE : TILE_RSPLIT;
+    const int plain_mode = inverse ? TILE_ROWS_TO_COLS : TILE_COLS_TWID;
+    long long bestf = 0;
+    for (const TileVariant &v : tile_registry()) {
+        if (v.mode != fused_mode || v.inverse != inverse || nc % v.n != 0)
+            continue;
+        const long long other = nc / v.n;
+        if (other < 32 || other % 2 || (other / 2) % (v.TL / 2))
+            continue;
+        bool have = false;
+        for (const TileVariant &w : tile_registry())
+            have |= w.mode == plain_mode && w.inverse == inverse && w.n == other;
+        if (!have)
+            continue;
+        auto score = [&](long long f) { return f > nc / f ? f : nc / f; };
+        if (bestf == 0 || score(v.n) < score(bestf))
+            bestf = v.n;
+    }
+    if (bestf == 0)
+        return 1;
+    const long long other = nc / bestf;
+    const long long N1 = inverse ? other : bestf, N2 = inverse ? bestf : other;
+    const TileChoice *cf = nullptr, *cp = nullptr;
+    int rc;
+    if ((rc = tile_choice((int)bestf, inverse, fused_mode, s, &cf)) ||
+        (rc = tile_choice((int)other, inverse, plain_mode, s, &cp)))
+        return rc;
+    if (!cf->v || !cp->v || (other / 2) % (cf->v->TL / 2))
+        return 1;
+    RealTileArgs r{};
+    r.nc = nc;
+    TileArgs a{};
+    if ((rc = fourstep_twiddles(nc, inverse, &a.tw_hi, &a.tw_lo)) ||
+        (rc = real_split_twiddles(nc, (int)bestf, inverse, &r.rtw_a, &r.rtw_b)))
+        return rc;
+    if (!inverse) {
+        // pass 1: line (b, n1): element n2 at in[b*in_hi + n1 + N1*n2] -> tmp[b*nc + n1 + N1*k2]
+        a.in = in;
+        a.out = tmp;
+        a.nlines = howmany * N1;
+        a.nlo = N1;
+        a.in_hi = in_hi;
+        a.in_lo = 1;
+        a.in_el = N1;
+        a.out_hi = nc;
+        a.out_lo = 1;
+        a.out_el = N1;
+        rc = launch_tile(*cp, a, s);
+        // pass 2: rows k2 and N2-k2 of tmp -> bins
+        r.in = tmp;
+        r.out = out;
+        r.in_hi = nc;
+        r.out_hi = out_hi;
+        r.other = (int)N2;
+        if (rc == 0)
+            rc = launch_rtile(*cf, r, howmany, s);
+    } else {
+        // pass 1: column pairs (n1, N1-n1) of the bins -> tmp[b*tp + n1 + N1*k2].  Each
+        // transform of tmp sits at the same offset in its 128-byte line as its bins, so
+        // the kernel's line-aligned runs are aligned on both sides (tp = in_hi mod 16).
+        const long long tp = nc + ((in_hi - nc) & 15);
+        const uintptr_t want = (reinterpret_cast<uintptr_t>(in) / 8) & 15;
+        void *tmp_al = (char *)tmp + 8 * ((want - reinterpret_cast<uintptr_t>(tmp) / 8) & 15);
+        r.in = in;
+        r.out = tmp_al;
+        r.tw_hi = a.tw_hi;
+        r.tw_lo = a.tw_lo;
+        r.in_hi = in_hi;
+        r.out_hi = tp;
+        r.other = (int)N1;
+        rc = launch_rtile(*cf, r, howmany, s);
+        // pass 2: line (b, k2): element n1 at tmp[b*tp + n1 + N1*k2] -> z[b*out_hi + k2 + N2*k1]
+        a.in = tmp_al;
+        a.out = out;
+        a.nlines = howmany * N2;
+        a.nlo = N2;
+        a.in_hi = tp;
+        a.in_lo = N1;
+        a.in_el = 1;
+        a.out_hi = out_hi;
+        a.out_lo = 1;
+        a.out_el = N2;
+        if (rc == 0)
+            rc = launch_tile(*cp, a, s);
+    }
+    return rc;
+}
+
+// Real rows whose half length needs the multi-pass complex transform.  Float lengths with
+// a registered four-step split take the two fused passes of run_real_fourstep; otherwise
+// the complex pass runs into a stream-ordered temporary (nc values per line) and the
+// split (forward) or merge (inverse) runs as its own HBM pass on that temporary.
 static int run_real_multipass(DevPlan &H, const RealTables &R, int inverse, const void *in, long long in_hi,
                               void *out, long long out_hi, long long howmany, hipStream_t s)
 {
@@ -1184,8 +1367,8 @@ static int run_real_multipass(DevPlan &H, const RealTables &R, int inverse, cons
         }
         return 0;
     }
-    void *tmp = nullptr;
-    KFA_HIP(scratch_alloc(&tmp, cb * (size_t)nc * (size_t)howmany, s));
+    void *tmp = nullptr;  // room for the fused inverse's aligned layout (16 more values per transform)
+    KFA_HIP(scratch_alloc(&tmp, cb * ((size_t)(nc + 16) * (size_t)howmany + 16), s));
     auto pass = [&](int merge, const void *src, void *dst, long long hi) {
         switch (H.type) {
         case KFA_F32: return launch_real_pass<F32>(merge, src, dst, R.d_stw, nc, hi, howmany, s);
@@ -1193,7 +1376,11 @@ static int run_real_multipass(DevPlan &H, const RealTables &R, int inverse, cons
         default: return launch_real_pass<Q31>(merge, src, dst, R.d_stw, nc, hi, howmany, s);
         }
     };
-    int rc;
+    int rc = run_real_fourstep(H, R, inverse, in, in_hi, out, out_hi, howmany, tmp, s);
+    if (rc != 1) {
+        (void)hipFreeAsync(tmp, s);
+        return rc;
+    }
     if (!inverse) {
         rc = run_c2c(H, in, 1, in_hi, tmp, 1, nc, howmany, s);
         if (rc == 0)
@@ -1326,7 +1513,8 @@ int kfa_c2c(const kfa_cplan *pl, const void *in, long long istride, long long id
     const size_t in_bytes = cb * (size_t)((howmany - 1) * idist + (n - 1) * istride + 1);
     const size_t out_bytes = cb * (size_t)((howmany - 1) * odist + n);
     Operand I, O;
-    if ((rc = stage_in(in, in_bytes, s, I)) == 0 && (rc = stage_out(out, out_bytes, s, O)) == 0) {
+    if ((rc = stage_in(in, in_bytes, s, I)) == 0 &&
+        (rc = stage_out(out, out_bytes, s, O, howmany > 1 && odist > n)) == 0) {
         const bool same = I.dev == O.dev && !I.staged && !O.staged;
         const bool inplace_ok = !same || (P->single && istride == 1 && idist == odist);
         if (inplace_ok) {
@@ -1375,7 +1563,8 @@ static int real_common(const kfa_rplan *pl, const void *in, long long idist, voi
     Operand I, O;
     rc = stage_in(in, inverse ? cpx_bytes_in : real_bytes_in, s, I);
     if (rc == 0)
-        rc = stage_out(out, inverse ? real_bytes_out : cpx_bytes_out, s, O);
+        rc = stage_out(out, inverse ? real_bytes_out : cpx_bytes_out, s, O,
+                       howmany > 1 && odist > (inverse ? 2 * nc : nc + 1));
     if (rc == 0) {
         if (!inverse && (idist & 1))
             rc = fail(-2, "real input distance must be even (complex-pair loads)");
@@ -1634,12 +1823,17 @@ void kfa_cleanup(void)
     for (auto &kv : T.choice)
         if (kv.second.d_stw)
             (void)hipFree(kv.second.d_stw);
-    for (auto &kv : T.fs) {
-        (void)hipFree(kv.second.first);
-        (void)hipFree(kv.second.second);
-    }
+    auto free_pairs = [](auto &m) {
+        for (auto &kv : m) {
+            (void)hipFree(kv.second.first);
+            (void)hipFree(kv.second.second);
+        }
+    };
+    free_pairs(T.fs);
+    free_pairs(T.rs);
     T.choice.clear();
     T.fs.clear();
+    T.rs.clear();
     blue_release();
     conv_release();
 }

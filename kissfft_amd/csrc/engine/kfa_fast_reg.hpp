@@ -33,9 +33,11 @@ const std::vector<FastVariant> &fast_registry();
 namespace kfa {
 
 struct TileArgs;
+struct RealTileArgs;
 
-// Tile-kernel variants (kfa_tile.hpp): float, one length, direction and HBM mapping.
-enum { TILE_COLS = 0, TILE_COLS_TWID = 1, TILE_ROWS_TO_COLS = 2 };
+// Tile-kernel variants (kfa_tile.hpp, kfa_tile_real.hpp): float, one length, direction
+// and HBM mapping.
+enum { TILE_COLS = 0, TILE_COLS_TWID = 1, TILE_ROWS_TO_COLS = 2, TILE_RSPLIT = 3, TILE_RMERGE = 4 };
 
 struct TileVariant {
     int n;
@@ -43,13 +45,16 @@ struct TileVariant {
     int mode;     // TILE_COLS: column load / column store (N-D axes)
                   // TILE_COLS_TWID: the same with the four-step twiddle (first pass)
                   // TILE_ROWS_TO_COLS: row load / column store (four-step second pass)
+                  // TILE_RSPLIT: row pairs, real split fused (forward real second pass)
+                  // TILE_RMERGE: column pairs, real merge fused (inverse real first pass)
     int T, TL;    // threads per line, lines per workgroup
     std::vector<int> radices;
     int tw_pad;
     int tw_len;
     size_t lds;   // dynamic LDS bytes
     const char *name;
-    int (*launch)(const TileArgs &, long long blocks, hipStream_t);
+    int (*launch)(const TileArgs &, long long blocks, hipStream_t);            // modes 0-2
+    int (*launch_real)(const RealTileArgs &, long long blocks, hipStream_t);  // modes 3-4
 };
 
 const std::vector<TileVariant> &tile_registry();

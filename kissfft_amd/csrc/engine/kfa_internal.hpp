@@ -78,7 +78,9 @@ struct Operand {
     bool staged = false;
 };
 int stage_in(const void *p, size_t bytes, hipStream_t s, Operand &op);
-int stage_out(void *p, size_t bytes, hipStream_t s, Operand &op);
+// gaps: the output span holds values the call must not touch (rows further apart than
+// their length); a staged copy then starts from the caller's bytes.
+int stage_out(void *p, size_t bytes, hipStream_t s, Operand &op, bool gaps = false);
 int finish_out(Operand &op, hipStream_t s);
 void release(Operand &op, hipStream_t s);
 

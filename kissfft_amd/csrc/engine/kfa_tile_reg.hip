@@ -4,7 +4,8 @@
 // Lengths cover the non-last axes of the N-D workloads (64, 96, 1024, ...) and the halves
 // of the four-step split of long lines (128 .. 1024, N = N1 * N2 up to 2^20 and beyond).
 // Each length is registered in three HBM mappings (column / column + twiddle / row-in
-// column-out) and both directions.
+// column-out) and both directions; the halves of long real transforms also have the
+// fused real split (forward) and merge (inverse) passes.
 #include <hip/hip_runtime.h>
 
 #include <mutex>
@@ -12,6 +13,7 @@
 #include "kfa_fast_reg.hpp"
 #include "kfa_blue.hpp"
 #include "kfa_tile.hpp"
+#include "kfa_tile_real.hpp"
 
 namespace kfa {
 
@@ -51,6 +53,7 @@ static void add_tile(std::vector<TileVariant> &v, const char *name)
             f.tw_len = RS::tw_len();
             f.lds = (size_t)TL * (RS::N() + 1) * 8;
             f.name = name;
+            f.launch_real = nullptr;
             if (mode == TILE_COLS)
                 f.launch = inv ? &launch_tile<true, T, TL, true, true, false, RS>
                                : &launch_tile<false, T, TL, true, true, false, RS>;
@@ -62,6 +65,50 @@ static void add_tile(std::vector<TileVariant> &v, const char *name)
                                : &launch_tile<false, T, TL, false, true, false, RS>;
             v.push_back(f);
         }
+}
+
+template <bool MERGE, int T, int TL, class RS>
+static int launch_rtile(const RealTileArgs &a, long long blocks, hipStream_t s)
+{
+    const size_t lds = (size_t)TL * (RS::N() + 1) * sizeof(float2);
+    const void *fn = MERGE ? (const void *)k_tile_rmerge<T, TL, RS> : (const void *)k_tile_rsplit<T, TL, RS>;
+    static std::once_flag once;
+    std::call_once(once, [=] { (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); });
+    if (blocks <= 0)
+        return 0;
+    if (MERGE)
+        hipLaunchKernelGGL((k_tile_rmerge<T, TL, RS>), dim3((unsigned)blocks), dim3(T * TL), lds, s, a);
+    else
+        hipLaunchKernelGGL((k_tile_rsplit<T, TL, RS>), dim3((unsigned)blocks), dim3(T * TL), lds, s, a);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+// The fused real passes of one length: split (forward, row pairs) and merge (inverse,
+// column pairs).
+template <int T, int TL, int... R>
+static void add_rtile(std::vector<TileVariant> &v, const char *name)
+{
+    constexpr int G = 2;
+    typedef Radices<G, R...> RS;
+    static_assert(T * TL <= 1024, "workgroup too large");
+    static_assert((size_t)TL * (RS::N() + 1) * 8 + (size_t)TL * (RS::N() / T) * 8 + 8 <= 160 * 1024,
+                  "tile exceeds LDS");
+    for (int inv = 0; inv < 2; ++inv) {
+        TileVariant f;
+        f.n = RS::N();
+        f.inverse = inv;
+        f.mode = inv ? TILE_RMERGE : TILE_RSPLIT;
+        f.T = T;
+        f.TL = TL;
+        f.radices = {R...};
+        f.tw_pad = G;
+        f.tw_len = RS::tw_len();
+        f.lds = (size_t)TL * (RS::N() + 1) * 8;
+        f.name = name;
+        f.launch = nullptr;
+        f.launch_real = inv ? &launch_rtile<true, T, TL, RS> : &launch_rtile<false, T, TL, RS>;
+        v.push_back(f);
+    }
 }
 
 const std::vector<TileVariant> &tile_registry()
@@ -80,6 +127,13 @@ const std::vector<TileVariant> &tile_registry()
         add_tile<16, 32, 16, 16>(*v, "tile256_1616_t16x32");
         add_tile<16, 16, 8, 16>(*v, "tile128_816_t16x16");
         add_tile<8, 32, 8, 8>(*v, "tile64_88_t8x32");
+        add_rtile<16, 32, 8, 16>(*v, "rtile128_816_t16x32");
+        add_rtile<16, 16, 16, 16>(*v, "rtile256_1616_t16x16");
+        add_rtile<16, 32, 16, 16>(*v, "rtile256_1616_t16x32");
+        add_rtile<64, 16, 8, 8, 8>(*v, "rtile512_888_t64x16");
+        add_rtile<64, 8, 8, 8, 8>(*v, "rtile512_888_t64x8");
+        add_rtile<64, 16, 16, 16, 4>(*v, "rtile1024_16164_t64x16");
+        add_rtile<64, 8, 16, 16, 4>(*v, "rtile1024_16164_t64x8");
         return v;
     }();
     return *reg;

@@ -154,11 +154,12 @@ def test_float_c2c_long_lines_strided(lf):
     rng = np.random.default_rng(17)
     idist, odist = n * s + 5, n + 3
     buf = rng.uniform(-1, 1, (howmany * idist, 2)).astype(np.float32)
-    out = np.zeros((howmany * odist, 2), np.float32)
+    out = np.full((howmany * odist, 2), 9.0, np.float32)
     lf.fft_batch(lf.fft_alloc(n), buf, out, howmany, idist, odist, in_stride=s)
     for b in range(howmany):
         xb = cplx(buf[b * idist: b * idist + n * s: s])
         assert rel_l2(cplx(out[b * odist: b * odist + n]), np.fft.fft(xb)) <= ABS_FLOAT_BOUND, b
+        assert (out[b * odist + n: (b + 1) * odist] == 9.0).all(), b  # the gaps keep their values
 
 
 def test_float_stride_and_in_place(lf):

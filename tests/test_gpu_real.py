@@ -80,11 +80,16 @@ def test_fixed_real_multipass_bit_exact_vs_oracle(datatype, n):
     np.testing.assert_array_equal(c2r(lib, n, X), O.fftri(X, n))
 
 
+@pytest.mark.parametrize("fused", [True, False], ids=["fused", "unfused"])
 @pytest.mark.parametrize("n,howmany", [(32768, 8), (65536, 4), (1 << 20, 2), (3 * 5 * 7 * 1024, 2)])
-def test_float_real_multipass(n, howmany):
+def test_float_real_multipass(n, howmany, fused, monkeypatch):
+    """Long real lines: the split / merge fused into the four-step passes (when a split of
+    n/2 is registered; 107520 has none and always takes the three-pass form) and the
+    three-pass form (complex transform, then split / merge as a pass of its own)."""
+    monkeypatch.setenv("KISSFFT_AMD_REAL_FUSED", "1" if fused else "0")
     lib = kissfft_amd.load("float")
     x = W.two_tone(5, n, howmany)
-    say("case float real multipass", n)
+    say("case float real multipass", n, "fused" if fused else "unfused")
     X = r2c(lib, n, x)
     want = np.fft.rfft(x.astype(np.float64), axis=-1)
     err = rel_l2(cplx(X), want)
@@ -95,6 +100,27 @@ def test_float_real_multipass(n, howmany):
     err = rel_l2(y, np.fft.irfft(cplx(Xin), n=n, axis=-1) * n)
     say("c2r rel-L2", err)
     assert err <= ABS_FLOAT_BOUND
+
+
+@pytest.mark.parametrize("n", [65536, 1 << 20])
+def test_float_real_multipass_padded_rows(n):
+    """Rows further apart than one transform on both sides (idist, odist padded)."""
+    lib = kissfft_amd.load("float")
+    howmany, ipad, opad = 3, 6, 5
+    x = np.zeros((howmany, n + ipad), np.float32)
+    x[:, :n] = W.two_tone(9, n, howmany)
+    say("case float real padded rows", n)
+    X = np.full((howmany, n // 2 + 1 + opad, 2), 7.0, np.float32)
+    lib.fftr_batch(lib.fftr_alloc(n, False), x, X, howmany, n + ipad, n // 2 + 1 + opad)
+    want = np.fft.rfft(x[:, :n].astype(np.float64), axis=-1)
+    assert rel_l2(cplx(X[:, : n // 2 + 1]), want) <= ABS_FLOAT_BOUND
+    assert (X[:, n // 2 + 1 :] == 7.0).all()
+    Xin = np.zeros((howmany, n // 2 + 1 + opad, 2), np.float32)
+    Xin[:, : n // 2 + 1] = pairs(want.astype(np.complex64))
+    y = np.full((howmany, n + ipad), 5.0, np.float32)
+    lib.fftri_batch(lib.fftr_alloc(n, True), Xin, y, howmany, n // 2 + 1 + opad, n + ipad)
+    assert rel_l2(y[:, :n], np.fft.irfft(cplx(Xin[:, : n // 2 + 1]), n=n, axis=-1) * n) <= ABS_FLOAT_BOUND
+    assert (y[:, n:] == 5.0).all()
 
 
 @pytest.mark.parametrize("n", [4096, 4000])

@@ -14,7 +14,7 @@ import pytest
 
 import kissfft_amd
 from conftest import ROOT
-from helpers import ABS_FLOAT_BOUND, cplx, rel_l2
+from helpers import ABS_FLOAT_BOUND, cplx, pairs, rel_l2
 from kissfft_amd import workloads as W
 from oracle import Oracle
 
@@ -153,5 +153,44 @@ def test_forced_tile(n, name, monkeypatch):
                 lib.fft_batch(lib.fft_alloc(N, inv), xs, ys, 2, N, N)
                 want = np.fft.ifft(cplx(xs), axis=-1) * N if inv else np.fft.fft(cplx(xs), axis=-1)
                 assert rel_l2(cplx(ys), want) <= ABS_FLOAT_BOUND, (name, inv)
+    finally:
+        lib.L.kiss_fft_cleanup()
+
+
+def rtile_registry():
+    out = []
+    for m in re.finditer(r"add_rtile<([^>]*)>\(\*v,\s*\"(\w+)\"\)", open(TREG).read()):
+        a = [int(x) for x in m.group(1).split(",")]
+        out.append((int(np.prod(a[2:])), m.group(2)))
+    return out
+
+
+RTILES = rtile_registry()
+
+
+def test_rtile_registry_parsed():
+    assert len(RTILES) >= 4
+
+
+@pytest.mark.parametrize("n,name", RTILES, ids=[t[1] for t in RTILES])
+def test_forced_rtile(n, name, monkeypatch):
+    """Each fused real split / merge variant forced, on real lines of 2*n*n points (half
+    length n*n, split n x n)."""
+    lib = kissfft_amd.load("float")
+    monkeypatch.setenv("KISSFFT_AMD_TILE", name)
+    monkeypatch.setenv("KISSFFT_AMD_VERBOSE", "1")
+    lib.L.kiss_fft_cleanup()
+    try:
+        N = 2 * n * n
+        x = W.two_tone(72, N, 3)
+        say("rtile", name, "real", N)
+        X = np.zeros((3, N // 2 + 1, 2), np.float32)
+        lib.fftr_batch(lib.fftr_alloc(N, False), x, X, 3, N, N // 2 + 1)
+        want = np.fft.rfft(x.astype(np.float64), axis=-1)
+        assert rel_l2(cplx(X), want) <= ABS_FLOAT_BOUND, name
+        Xin = pairs(want.astype(np.complex64))
+        y = np.zeros((3, N), np.float32)
+        lib.fftri_batch(lib.fftr_alloc(N, True), Xin, y, 3, N // 2 + 1, N)
+        assert rel_l2(y, np.fft.irfft(cplx(Xin), n=N, axis=-1) * N) <= ABS_FLOAT_BOUND, name
     finally:
         lib.L.kiss_fft_cleanup()
