@@ -35,6 +35,7 @@
 #include "kfa_blue.hpp"
 #include "kfa_tile.hpp"
 #include "kfa_tile_real.hpp"
+#include "kfa_plane.hpp"
 
 namespace kfa {
 
@@ -352,11 +353,17 @@ struct TileChoice {
     void *d_stw = nullptr;
 };
 
+struct PlaneChoice {
+    const PlaneVariant *v = nullptr;
+    void *stw_a = nullptr, *stw_b = nullptr;
+};
+
 struct TileCache {
     std::mutex mu;
     std::map<std::tuple<int, int, int, int>, TileChoice> choice;            // dev, n, inv, mode
     std::map<std::tuple<int, long long, int>, std::pair<void *, void *>> fs; // dev, N, inv -> hi, lo
     std::map<std::tuple<int, long long, int, int>, std::pair<void *, void *>> rs;  // dev, nc, n, inv
+    std::map<std::tuple<int, int, int, int>, PlaneChoice> plane;                  // dev, A, B, inv
 };
 
 static TileCache &tile_cache()
@@ -753,6 +760,100 @@ static void blue_release()
                 (void)hipFree(p);
     }
     g_blue.clear();
+}
+
+// The plane variant for the last two axes A x B (float) on the current device, or
+// v == nullptr.  Several candidates are timed once on about 128 MiB of planes
+// (KISSFFT_AMD_PLANE=0 turns the plane pass off, KISSFFT_AMD_TILE forces a variant by name).
+static int plane_choice(int A, int B, int inverse, hipStream_t s, const PlaneChoice **out)
+{
+    int dev;
+    int rc = current_device(&dev);
+    if (rc)
+        return rc;
+    auto &C = tile_cache();
+    std::lock_guard<std::mutex> g(C.mu);
+    const auto key = std::make_tuple(dev, A, B, inverse);
+    auto it = C.plane.find(key);
+    if (it != C.plane.end()) {
+        *out = &it->second;
+        return 0;
+    }
+    std::vector<PlaneChoice> cand;
+    if (env_flag("KISSFFT_AMD_FAST", true) && env_flag("KISSFFT_AMD_PLANE", true)) {
+        const char *force = getenv("KISSFFT_AMD_TILE");
+        bool named = false;
+        for (const PlaneVariant &v : plane_registry())
+            named |= force && *force && !strcmp(force, v.name) && v.A == A && v.B == B;
+        for (const PlaneVariant &v : plane_registry())
+            if (v.A == A && v.B == B && v.inverse == inverse && (!named || !strcmp(force, v.name)))
+                cand.push_back(PlaneChoice{&v, nullptr, nullptr});
+    }
+    for (PlaneChoice &c : cand) {
+        const std::vector<unsigned char> ta = float_twiddles(A, inverse, A, 1), tb = float_twiddles(B, inverse, B, 1);
+        if ((rc = make_stage_table_raw(ta.data(), KFA_F32, A, c.v->rad_a, c.v->tw_pad, 0, &c.stw_a)) ||
+            (rc = make_stage_table_raw(tb.data(), KFA_F32, B, c.v->rad_b, c.v->tw_pad, 0, &c.stw_b)))
+            return rc;
+    }
+    size_t best = 0;
+    if (cand.size() > 1 && env_flag("KISSFFT_AMD_TUNE", true)) {
+        const long long planes = std::max(1LL, (16LL << 20) / ((long long)A * B));
+        const size_t bytes = 8 * (size_t)A * B * (size_t)planes;
+        void *bin = nullptr, *bout = nullptr;
+        KFA_HIP(scratch_alloc(&bin, bytes, s));
+        KFA_HIP(scratch_alloc(&bout, bytes, s));
+        if ((rc = fill_tune(bin, bytes, KFA_F32, s)))
+            return rc;
+        hipEvent_t e0, e1;
+        KFA_HIP(hipEventCreate(&e0));
+        KFA_HIP(hipEventCreate(&e1));
+        std::vector<float> t(cand.size(), 1e30f);
+        auto run = [&](size_t i) {
+            PlaneArgs pa{bin, bout, cand[i].stw_a, cand[i].stw_b, planes};
+            return cand[i].v->launch(pa, s);
+        };
+        for (size_t i = 0; i < cand.size() && rc == 0; ++i)
+            rc = run(i);
+        for (int round = 0; round < 3 && rc == 0; ++round)
+            for (size_t i = 0; i < cand.size() && rc == 0; ++i) {
+                KFA_HIP(hipEventRecord(e0, s));
+                for (int r = 0; r < 3 && rc == 0; ++r)
+                    rc = run(i);
+                KFA_HIP(hipEventRecord(e1, s));
+                KFA_HIP(hipEventSynchronize(e1));
+                float ms = 0;
+                KFA_HIP(hipEventElapsedTime(&ms, e0, e1));
+                t[i] = std::min(t[i], ms);
+            }
+        for (size_t i = 1; i < cand.size(); ++i)
+            if (t[i] < 0.98f * t[best])
+                best = i;
+        if (env_flag("KISSFFT_AMD_VERBOSE", false))
+            for (size_t i = 0; i < cand.size(); ++i)
+                fprintf(stderr, "[kissfft_amd]   %-32s plane %8.1f us\n", cand[i].v->name, 1e3f * t[i] / 3);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipFreeAsync(bin, s);
+        (void)hipFreeAsync(bout, s);
+        KFA_HIP(hipStreamSynchronize(s));
+        if (rc)
+            return fail(-3, "plane kernel launch failed");
+    }
+    PlaneChoice pc;
+    for (size_t i = 0; i < cand.size(); ++i) {
+        if (i == best) {
+            pc = cand[i];
+        } else {
+            (void)hipFree(cand[i].stw_a);
+            (void)hipFree(cand[i].stw_b);
+        }
+    }
+    if (pc.v && env_flag("KISSFFT_AMD_VERBOSE", false))
+        fprintf(stderr, "[kissfft_amd] plane %dx%d inverse=%d: %s\n", A, B, inverse, pc.v->name);
+    auto &slot = C.plane[key];
+    slot = pc;
+    *out = &slot;
+    return 0;
 }
 
 // Float four-step (Bailey) for n = N1*N2 with both halves registered as tile variants:
@@ -1657,7 +1758,22 @@ int run_nd(const std::vector<std::shared_ptr<DevPlan>> &P, const int *dims, cons
     long long outer = howmany, inner = total;
     const void *src = in;
     int rc = 0;
-    for (int k = 0; k < ndims && rc == 0; ++k) {
+    int last = ndims;  // axes [0, last) run one pass each
+    if (ndims >= 2 && P[ndims - 1]->type == KFA_F32) {
+        // the last two axes in one pass when a plane of them fits in LDS
+        const PlaneChoice *pc = nullptr;
+        if ((rc = plane_choice(dims[ndims - 2], dims[ndims - 1], P[ndims - 1]->inverse, s, &pc)))
+            return rc;
+        if (pc->v) {
+            PlaneArgs pa{in, out, pc->stw_a, pc->stw_b, howmany * (total / ((long long)dims[ndims - 2] * dims[ndims - 1]))};
+            ++g_launches;
+            if (pc->v->launch(pa, s))
+                return fail(-3, "plane kernel launch failed");
+            src = out;
+            last = ndims - 2;
+        }
+    }
+    for (int k = 0; k < last && rc == 0; ++k) {
         inner /= dims[k];
         rc = run_axis(*P[k], src, out, outer, inner, s);
         outer *= dims[k];
@@ -1831,6 +1947,11 @@ void kfa_cleanup(void)
     };
     free_pairs(T.fs);
     free_pairs(T.rs);
+    for (auto &kv : T.plane)
+        for (void *p : {kv.second.stw_a, kv.second.stw_b})
+            if (p)
+                (void)hipFree(p);
+    T.plane.clear();
     T.choice.clear();
     T.fs.clear();
     T.rs.clear();

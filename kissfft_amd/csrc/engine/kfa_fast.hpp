@@ -366,7 +366,7 @@ struct FastStage {
                 if (GSRC)
                     v[j][r] = valid ? O::load(hbm_ld(src + idx * src_el)) : O::zero();
                 else
-                    v[j][r] = O::load(src[idx]);
+                    v[j][r] = O::load(src[idx * src_el]);
             }
         }
     }
@@ -502,7 +502,7 @@ struct FastStage {
                     if (valid)
                         hbm_st(dst + idx * dst_el, O::pack(v[j][q]));
                 } else {
-                    dst[idx] = O::pack(v[j][q]);
+                    dst[idx * dst_el] = O::pack(v[j][q]);
                 }
             }
         }

@@ -99,3 +99,24 @@ struct ConvVariant {
 const std::vector<ConvVariant> &conv_registry();
 
 }  // namespace kfa
+
+namespace kfa {
+
+struct PlaneArgs;
+
+// Plane variants (kfa_plane.hpp): float, the last two axes A x B of an N-D transform in
+// one pass, one direction.
+struct PlaneVariant {
+    int A, B;
+    int inverse;
+    int TA, TB, NTH;  // threads per column, per row, per workgroup
+    std::vector<int> rad_a, rad_b;
+    int tw_pad;
+    size_t lds;
+    const char *name;
+    int (*launch)(const PlaneArgs &, hipStream_t);
+};
+
+const std::vector<PlaneVariant> &plane_registry();
+
+}  // namespace kfa

@@ -14,6 +14,7 @@
 #include "kfa_blue.hpp"
 #include "kfa_tile.hpp"
 #include "kfa_tile_real.hpp"
+#include "kfa_plane.hpp"
 
 namespace kfa {
 
@@ -127,6 +128,8 @@ const std::vector<TileVariant> &tile_registry()
         add_tile<16, 32, 16, 16>(*v, "tile256_1616_t16x32");
         add_tile<16, 16, 8, 16>(*v, "tile128_816_t16x16");
         add_tile<8, 32, 8, 8>(*v, "tile64_88_t8x32");
+        add_tile<4, 64, 8, 4>(*v, "tile32_84_t4x64");
+        add_tile<1, 256, 16>(*v, "tile16_16_t1x256");
         add_rtile<16, 32, 8, 16>(*v, "rtile128_816_t16x32");
         add_rtile<16, 16, 16, 16>(*v, "rtile256_1616_t16x16");
         add_rtile<16, 32, 16, 16>(*v, "rtile256_1616_t16x32");
@@ -134,6 +137,85 @@ const std::vector<TileVariant> &tile_registry()
         add_rtile<64, 8, 8, 8, 8>(*v, "rtile512_888_t64x8");
         add_rtile<64, 16, 16, 16, 4>(*v, "rtile1024_16164_t64x16");
         add_rtile<64, 8, 16, 16, 4>(*v, "rtile1024_16164_t64x8");
+        return v;
+    }();
+    return *reg;
+}
+
+template <class RA, class RB>
+constexpr size_t plane_lds()
+{
+    constexpr int B = RB::N();
+    return ((size_t)RA::N() * (B % 2 == 0 ? B + 1 : B) + RA::tw_len() + RB::tw_len()) * sizeof(float2);
+}
+
+// Planes of which one workgroup fills a CU's LDS run persistently (as many workgroups as
+// fit on the device, each looping over its share of the planes, so that loads of the next
+// plane overlap the transform of the current one); smaller planes run one per workgroup,
+// several workgroups per CU overlapping each other.
+template <bool INV, int TA, int TB, int NTH, class RA, class RB>
+static int launch_plane(const PlaneArgs &a, hipStream_t s)
+{
+    const void *fn = (const void *)k_plane<INV, TA, TB, NTH, RA, RB>;
+    constexpr size_t lds = plane_lds<RA, RB>();
+    static long long resident = 0;  // 0: one workgroup per plane
+    static std::once_flag once;
+    std::call_once(once, [=] {
+        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        int dev = 0, cus = 0, per = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, NTH, lds) == hipSuccess && per == 1 && cus > 0)
+            resident = cus;
+    });
+    if (a.planes <= 0)
+        return 0;
+    const long long grid = (resident > 0 && a.planes > resident) ? resident : a.planes;
+    if (grid > 0x7fffffffLL)
+        return -2;
+    hipLaunchKernelGGL((k_plane<INV, TA, TB, NTH, RA, RB>), dim3((unsigned)grid), dim3(NTH), lds, s, a);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+template <class RS>
+static std::vector<int> radix_list()
+{
+    return std::vector<int>(RS::r, RS::r + RS::K);
+}
+
+template <int TA, int TB, int NTH, class RA, class RB>
+static void add_plane(std::vector<PlaneVariant> &v, const char *name)
+{
+    constexpr int B = RB::N();
+    constexpr size_t lds = plane_lds<RA, RB>();
+    static_assert(lds <= 160 * 1024, "plane exceeds LDS");
+    static_assert(NTH <= 1024, "workgroup too large");
+    for (int inv = 0; inv < 2; ++inv) {
+        PlaneVariant f;
+        f.A = RA::N();
+        f.B = B;
+        f.inverse = inv;
+        f.TA = TA;
+        f.TB = TB;
+        f.NTH = NTH;
+        f.rad_a = radix_list<RA>();
+        f.rad_b = radix_list<RB>();
+        f.tw_pad = 2;
+        f.lds = lds;
+        f.name = name;
+        f.launch = inv ? &launch_plane<true, TA, TB, NTH, RA, RB> : &launch_plane<false, TA, TB, NTH, RA, RB>;
+        v.push_back(f);
+    }
+}
+
+const std::vector<PlaneVariant> &plane_registry()
+{
+    static std::vector<PlaneVariant> *reg = [] {
+        auto *v = new std::vector<PlaneVariant>();
+        add_plane<8, 8, 960, Radices<2, 8, 4, 3>, Radices<2, 8, 5, 3>>(*v, "plane96x120_t8x8");
+        add_plane<8, 8, 512, Radices<2, 8, 8>, Radices<2, 8, 8>>(*v, "plane64x64_t8x8");
+        add_plane<8, 8, 1024, Radices<2, 16, 8>, Radices<2, 16, 8>>(*v, "plane128x128_t8x8");
+        add_plane<4, 4, 128, Radices<2, 8, 4>, Radices<2, 8, 4>>(*v, "plane32x32_t4x4");
         return v;
     }();
     return *reg;

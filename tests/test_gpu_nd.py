@@ -1,10 +1,13 @@
 """GPU parity of the N-D paths (kiss_fftnd / kiss_fftndr / kiss_fftndri / batched forms)."""
+import os
+import re
 import sys
 
 import numpy as np
 import pytest
 
 import kissfft_amd
+from conftest import ROOT
 from helpers import ABS_FLOAT_BOUND, cplx, fixture_cases, float_bound, pairs, parse_dims, rel_l2
 from kissfft_amd import workloads as W
 from oracle import Oracle
@@ -157,4 +160,57 @@ def test_nd_in_place_single_call():
     want = np.fft.fftn(cplx(x))
     cfg = lib.fftnd_alloc(dims)
     lib.kiss_fftnd(cfg, x, x)
+    assert rel_l2(cplx(x), want) <= ABS_FLOAT_BOUND
+
+
+def plane_registry():
+    src = open(os.path.join(ROOT, "kissfft_amd", "csrc", "engine", "kfa_tile_reg.hip")).read()
+    out = []
+    for m in re.finditer(r"add_plane<\d+, \d+, \d+, Radices<2, ([\d, ]+)>, Radices<2, ([\d, ]+)>>\(\*v, \"(\w+)\"\)", src):
+        A = int(np.prod([int(v) for v in m.group(1).split(",")]))
+        B = int(np.prod([int(v) for v in m.group(2).split(",")]))
+        out.append((A, B, m.group(3)))
+    return out
+
+
+PLANES = plane_registry()
+
+
+def test_plane_registry_parsed():
+    assert len(PLANES) >= 3 and (96, 120, "plane96x120_t8x8") in PLANES
+
+
+@pytest.mark.parametrize("A,B,name", PLANES, ids=[p[2] for p in PLANES])
+@pytest.mark.parametrize("lead", [(), (3,), (2, 5)], ids=["2d", "3d", "4d"])
+def test_float_nd_plane(A, B, name, lead, monkeypatch):
+    """The last two axes in one pass (plane kernel forced by name), under 0-2 leading axes,
+    both directions, against numpy; the same call with the plane pass off agrees too."""
+    lib = kissfft_amd.load("float")
+    dims = lead + (A, B)
+    x = W.complex_pm1(46, int(np.prod(dims)), 2).reshape((2,) + dims + (2,))
+    axes = tuple(range(1, len(dims) + 1))
+    monkeypatch.setenv("KISSFFT_AMD_TILE", name)
+    monkeypatch.setenv("KISSFFT_AMD_VERBOSE", "1")
+    lib.L.kiss_fft_cleanup()
+    try:
+        for inv in (False, True):
+            say("case nd plane", name, dims, inv)
+            want = np.fft.ifftn(cplx(x), axes=axes) * np.prod(dims) if inv else np.fft.fftn(cplx(x), axes=axes)
+            got = cplx(nd(lib, dims, x, inv))
+            assert rel_l2(got, want) <= ABS_FLOAT_BOUND, (dims, inv)
+        monkeypatch.setenv("KISSFFT_AMD_PLANE", "0")
+        lib.L.kiss_fft_cleanup()
+        got = cplx(nd(lib, dims, x))
+        assert rel_l2(got, np.fft.fftn(cplx(x), axes=axes)) <= ABS_FLOAT_BOUND, dims
+    finally:
+        lib.L.kiss_fft_cleanup()
+
+
+def test_float_nd_plane_in_place():
+    lib = kissfft_amd.load("float")
+    dims = (4, 96, 120)
+    x = W.complex_pm1(47, int(np.prod(dims)), 3).reshape((3,) + dims + (2,))
+    want = np.fft.fftn(cplx(x), axes=(1, 2, 3))
+    cfg = lib.fftnd_alloc(dims)
+    lib.fftnd_batch(cfg, x, x, 3)
     assert rel_l2(cplx(x), want) <= ABS_FLOAT_BOUND
