@@ -94,35 +94,18 @@ __global__ void __launch_bounds__(T *TL) k_tile_rsplit(RealTileArgs a)
     const int j0 = (int)(tile - b * tpt) * NP + bin_rotation(a.out, b, a.out_hi);
     const int tid = threadIdx.x;
 
-    {   // row mapping: T lanes per row, every load issued before the LDS writes
-        const int s = tid / T, t = tid - (tid / T) * T;
-        const int row = pair_line(rot_pair(j0, s >> 1, N2 / 2), s & 1, N2);
-        const St *src = reinterpret_cast<const St *>(a.in) + b * a.in_hi + (long long)row * N1 + t;
-        St v[IT];
-#pragma unroll
-        for (int k = 0; k < IT; ++k)
-            v[k] = src[k * T];
-#pragma unroll
-        for (int k = 0; k < IT; ++k)
-            buf[s * NS + t + k * T] = v[k];
-    }
-    __syncthreads();
-    {
-        const int line = tid / T, t = tid - line * T;
-        tile_stages<F32, false, T, RS>(std::make_integer_sequence<int, RS::K>{}, buf + line * NS,
-                                       reinterpret_cast<const St *>(a.stw), t);
-    }
     // Split item (j, k1): lanes over j, so a wave's bins are NP-long runs.  Pair 0: items
     // k1 < N1/2 take row 0's pairs (k1, N1-k1) -- k1 = 0 is DC / Nyquist, which also takes
     // the middle bin nc/2 (row 0, element N1/2) -- and items k1 >= N1/2 take row N2/2's
     // pairs (m, N1-1-m), m = k1 - N1/2.  Every other item takes Z[q + N2*k1] (row q) and
-    // its partner (row N2-q, element N1-1-k1).
+    // its partner (row N2-q, element N1-1-k1).  The items' LDS slots and twiddle reads are
+    // set up first, so that the table reads are in flight with the row loads.
     const St *rta = reinterpret_cast<const St *>(a.rtw_a);
     const St *rtb = reinterpret_cast<const St *>(a.rtw_b);
-    St *out = reinterpret_cast<St *>(a.out) + b * a.out_hi;
     const long long nc = a.nc;
-    C za[IS], zb[IS], ta[IS], tb[IS], zm{};
+    int s1[IS], s2[IS];  // LDS slots of Z[kl] and Z[nc-kl]
     long long kl[IS];
+    St ta[IS], tb[IS], tm0{}, tm1{};
 #pragma unroll
     for (int i = 0; i < IS; ++i) {
         const int w = tid + i * T * TL;
@@ -151,27 +134,49 @@ __global__ void __launch_bounds__(T *TL) k_tile_rsplit(RealTileArgs a)
                 e = N1 - 1 - k1;
             }
         }
-        const C z1 = O::load(buf[sa * NS + ea]), z2 = O::load(buf[sb * NS + eb]);
-        za[i] = swap ? z2 : z1;  // Z[min(k, nc-k)]
-        zb[i] = swap ? z1 : z2;
+        s1[i] = swap ? sb * NS + eb : sa * NS + ea;
+        s2[i] = swap ? sa * NS + ea : sb * NS + eb;
         kl[i] = q + (long long)N2 * e;
-        if (kl[i] == 0)  // Z[nc/2] = row 0, element N1/2: its own partner
-            zm = O::load(buf[sa * NS + N1 / 2]);
-        ta[i] = O::load(rta[q]);
-        tb[i] = O::load(rtb[e]);
+        ta[i] = rta[q];
+        tb[i] = rtb[e];
+        if (kl[i] == 0) {  // the middle bin's twiddle
+            tm0 = rta[0];
+            tm1 = rtb[N1 / 2];
+        }
     }
+    {   // row mapping: T lanes per row, every load issued before the LDS writes
+        const int s = tid / T, t = tid - (tid / T) * T;
+        const int row = pair_line(rot_pair(j0, s >> 1, N2 / 2), s & 1, N2);
+        const St *src = reinterpret_cast<const St *>(a.in) + b * a.in_hi + (long long)row * N1 + t;
+        St v[IT];
+#pragma unroll
+        for (int k = 0; k < IT; ++k)
+            v[k] = src[k * T];
+#pragma unroll
+        for (int k = 0; k < IT; ++k)
+            buf[s * NS + t + k * T] = v[k];
+    }
+    __syncthreads();
+    {
+        const int line = tid / T, t = tid - line * T;
+        tile_stages<F32, false, T, RS>(std::make_integer_sequence<int, RS::K>{}, buf + line * NS,
+                                       reinterpret_cast<const St *>(a.stw), t);
+    }
+    St *out = reinterpret_cast<St *>(a.out) + b * a.out_hi;
 #pragma unroll
     for (int i = 0; i < IS; ++i) {
+        const C za = O::load(buf[s1[i]]), zb = O::load(buf[s2[i]]);  // Z[kl], Z[nc-kl]
         C xa, xb;
         if (kl[i] == 0) {
-            split_dc<F32>(za[i], xa, xb);
+            split_dc<F32>(za, xa, xb);
             out[0] = O::pack(xa);
             out[nc] = O::pack(xb);
-            split_k<F32>(zm, zm, O::cmul(O::load(rta[0]), O::load(rtb[N1 / 2])), xa, xb);
+            const C zm = O::load(buf[s1[i] + N1 / 2]);  // Z[nc/2] = row 0, element N1/2: its own partner
+            split_k<F32>(zm, zm, O::cmul(O::load(tm0), O::load(tm1)), xa, xb);
             out[nc / 2] = O::pack(xb);
             continue;
         }
-        split_k<F32>(za[i], zb[i], O::cmul(ta[i], tb[i]), xa, xb);
+        split_k<F32>(za, zb, O::cmul(O::load(ta[i]), O::load(tb[i])), xa, xb);
         out[kl[i]] = O::pack(xa);
         out[nc - kl[i]] = O::pack(xb);
     }
@@ -208,10 +213,75 @@ __global__ void __launch_bounds__(T *TL) k_tile_rmerge(RealTileArgs a)
     const St *whi = reinterpret_cast<const St *>(a.tw_hi);
     const St *wlo = reinterpret_cast<const St *>(a.tw_lo);
 
+    // Twiddle reads first (they are in flight with the column loads): the per-line steps
+    // W^(col*T*k), this thread's store base W^(col*e0), and the merge items' factors.
+    constexpr int NSTEP = (TL * IT + T * TL - 1) / (T * TL);
+    St sh[NSTEP], sl[NSTEP];
+#pragma unroll
+    for (int i = 0; i < NSTEP; ++i) {
+        const int w = tid + i * T * TL;
+        if (w < TL * IT) {
+            const int s = w / IT, k = w - (w / IT) * IT;
+            const unsigned long long col = (unsigned)pair_line(rot_pair(j0, s >> 1, N1 / 2), s & 1, N1);
+            const unsigned long long ex = col * (unsigned)(k * T);  // < nc
+            sh[i] = whi[ex >> 10];
+            sl[i] = wlo[ex & 1023];
+        }
+    }
+    const int scol = pair_line(rot_pair(j0, (tid % TL) >> 1, N1 / 2), (tid % TL) & 1, N1);
+    St bh, bl;
+    {
+        const unsigned long long ex = (unsigned long long)scol * (unsigned)(tid / TL);
+        bh = whi[ex >> 10];
+        bl = wlo[ex & 1023];
+    }
+    // merge item (j, n2) owns the two LDS slots it reads (mirror of the split)
+    const St *rta = reinterpret_cast<const St *>(a.rtw_a);
+    const St *rtb = reinterpret_cast<const St *>(a.rtw_b);
+    St ta[IM], tb[IM], tm0{}, tm1{};
+    int ia[IM], ib[IM];  // LDS slots of X[kl] and X[nc-kl] (then z[kl], z[nc-kl])
+    bool dc[IM];
+#pragma unroll
+    for (int i = 0; i < IM; ++i) {
+        const int w = tid + i * T * TL;
+        const int j = w % NP, n2 = w / NP;
+        int ea, eb, sa = 2 * j, sb = 2 * j + 1;
+        int p = rot_pair(j0, j, N1 / 2), e = n2;  // the lower bin of the pair is p + N1*e
+        bool swap = false;
+        if (p == 0) {
+            if (2 * n2 < N2) {  // column 0
+                sb = sa;
+                ea = n2;
+                eb = n2 ? N2 - n2 : 0;
+            } else {            // column N1/2
+                sa = sb;
+                ea = n2 - N2 / 2;
+                eb = N2 - 1 - ea;
+                p = N1 / 2;
+                e = ea;
+            }
+        } else {
+            ea = n2;
+            eb = N2 - 1 - n2;
+            swap = 2 * (p + (long long)N1 * n2) > nc;
+            if (swap) {
+                p = N1 - p;
+                e = N2 - 1 - n2;
+            }
+        }
+        ia[i] = swap ? sb * NS + eb : sa * NS + ea;
+        ib[i] = swap ? sa * NS + ea : sb * NS + eb;
+        dc[i] = p == 0 && e == 0;
+        ta[i] = rta[p];
+        tb[i] = rtb[e];
+        if (dc[i]) {  // the middle bin's twiddle
+            tm0 = rta[0];
+            tm1 = rtb[N2 / 2];
+        }
+    }
     {   // column mapping: lanes over lines, every load issued before the LDS writes
         const int s = tid % TL, e0 = tid / TL;
-        const int col = pair_line(rot_pair(j0, s >> 1, N1 / 2), s & 1, N1);
-        const St *src = in + col + (long long)e0 * N1;
+        const St *src = in + scol + (long long)e0 * N1;
         St v[IT];
 #pragma unroll
         for (int k = 0; k < IT; ++k)
@@ -222,71 +292,33 @@ __global__ void __launch_bounds__(T *TL) k_tile_rmerge(RealTileArgs a)
         for (int k = 0; k < IT; ++k)
             buf[s * NS + e0 + k * T] = v[k];
     }
-    for (int w = tid; w < TL * IT; w += T * TL) {
-        const int s = w / IT, k = w - (w / IT) * IT;
-        const unsigned long long col = (unsigned)pair_line(rot_pair(j0, s >> 1, N1 / 2), s & 1, N1);
-        const unsigned long long ex = col * (unsigned)(k * T);  // < nc
-        s_step[w] = O::pack(O::cmul(O::load(whi[ex >> 10]), O::load(wlo[ex & 1023])));
+#pragma unroll
+    for (int i = 0; i < NSTEP; ++i) {
+        const int w = tid + i * T * TL;
+        if (w < TL * IT)
+            s_step[w] = O::pack(O::cmul(O::load(sh[i]), O::load(sl[i])));
     }
     __syncthreads();
-    {   // merge in place; item (j, n2) owns the two LDS slots it reads (mirror of the split)
-        const St *rta = reinterpret_cast<const St *>(a.rtw_a);
-        const St *rtb = reinterpret_cast<const St *>(a.rtw_b);
-        C xa[IM], xb[IM], ta[IM], tb[IM], xm{};
-        int ia[IM], ib[IM];  // LDS slots of z[kl] and z[nc-kl]
-        bool dc[IM];
+    {   // merge in place
+        C xa[IM], xb[IM], xm{};
 #pragma unroll
         for (int i = 0; i < IM; ++i) {
-            const int w = tid + i * T * TL;
-            const int j = w % NP, n2 = w / NP;
-            int ea, eb, sa = 2 * j, sb = 2 * j + 1;
-            int p = rot_pair(j0, j, N1 / 2), e = n2;  // the lower bin of the pair is p + N1*e
-            bool swap = false;
-            if (p == 0) {
-                if (2 * n2 < N2) {  // column 0
-                    sb = sa;
-                    ea = n2;
-                    eb = n2 ? N2 - n2 : 0;
-                } else {            // column N1/2
-                    sa = sb;
-                    ea = n2 - N2 / 2;
-                    eb = N2 - 1 - ea;
-                    p = N1 / 2;
-                    e = ea;
-                }
-            } else {
-                ea = n2;
-                eb = N2 - 1 - n2;
-                swap = 2 * (p + (long long)N1 * n2) > nc;
-                if (swap) {
-                    p = N1 - p;
-                    e = N2 - 1 - n2;
-                }
-            }
-            const int s1 = sa * NS + ea, s2 = sb * NS + eb;
-            const C x1 = O::load(buf[s1]), x2 = O::load(buf[s2]);
-            xa[i] = swap ? x2 : x1;  // X[min(k, nc-k)]
-            xb[i] = swap ? x1 : x2;
-            ia[i] = swap ? s2 : s1;
-            ib[i] = swap ? s1 : s2;
-            dc[i] = p == 0 && e == 0;
+            xa[i] = O::load(buf[ia[i]]);  // X[kl]
+            xb[i] = O::load(buf[ib[i]]);  // X[nc-kl]
             if (dc[i])  // z[nc/2] (column 0, element N2/2) pairs with itself; the DC item takes it
-                xm = O::load(buf[s1 + N2 / 2]);
-            ta[i] = O::load(rta[p]);
-            tb[i] = O::load(rtb[e]);
+                xm = O::load(buf[ia[i] + N2 / 2]);
         }
-        const C twm = O::cmul(O::load(rta[0]), O::load(rtb[N2 / 2]));
         __syncthreads();  // every operand read before any slot is rewritten
 #pragma unroll
         for (int i = 0; i < IM; ++i) {
             C zl, zh;
             if (dc[i]) {
                 buf[ia[i]] = O::pack(merge_dc<F32>(xa[i], O::load(s_nyq)));
-                merge_k<F32>(xm, xm, twm, zl, zh);
+                merge_k<F32>(xm, xm, O::cmul(O::load(tm0), O::load(tm1)), zl, zh);
                 buf[ia[i] + N2 / 2] = O::pack(zh);
                 continue;
             }
-            merge_k<F32>(xa[i], xb[i], O::cmul(ta[i], tb[i]), zl, zh);
+            merge_k<F32>(xa[i], xb[i], O::cmul(O::load(ta[i]), O::load(tb[i])), zl, zh);
             buf[ia[i]] = O::pack(zl);
             buf[ib[i]] = O::pack(zh);
         }
@@ -299,10 +331,8 @@ __global__ void __launch_bounds__(T *TL) k_tile_rmerge(RealTileArgs a)
     }
     {   // column stores with W^(col*k2), k2 = e0 + k*T
         const int s = tid % TL, e0 = tid / TL;
-        const int col = pair_line(rot_pair(j0, s >> 1, N1 / 2), s & 1, N1);
-        const unsigned long long ex = (unsigned long long)col * (unsigned)e0;
-        const C base = O::cmul(O::load(whi[ex >> 10]), O::load(wlo[ex & 1023]));
-        St *dst = reinterpret_cast<St *>(a.out) + b * a.out_hi + col + (long long)e0 * N1;
+        const C base = O::cmul(O::load(bh), O::load(bl));
+        St *dst = reinterpret_cast<St *>(a.out) + b * a.out_hi + scol + (long long)e0 * N1;
         C v[IT];
 #pragma unroll
         for (int k = 0; k < IT; ++k)
