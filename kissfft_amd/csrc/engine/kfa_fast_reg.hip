@@ -98,18 +98,20 @@ const std::vector<FastVariant> &fast_registry()
         auto *v = new std::vector<FastVariant>();
         // ---- float real transforms; the line is the half length ----
         // N = 4096 (half 2048)
+        // (full-size sweep, 65536 lines: t128x2_sb 5.0 TB/s both directions, the others
+        // 4.4 - 4.8)
+        add_real<F32, 128, 2, false, 16, 16, 8>(*v, KFA_F32, "real2048_16168_t128x2_sb");
         add_real<F32, 128, 1, false, 16, 16, 8>(*v, KFA_F32, "real2048_16168_t128_sb");
         add_real<F32, 256, 1, false, 8, 8, 8, 4>(*v, KFA_F32, "real2048_8884_t256_sb");
         add_real<F32, 256, 1, true, 8, 8, 8, 4>(*v, KFA_F32, "real2048_8884_t256_db");
         add_real<F32, 128, 1, true, 16, 16, 8>(*v, KFA_F32, "real2048_16168_t128_db");
         add_real<F32, 128, 1, false, 8, 16, 16>(*v, KFA_F32, "real2048_81616_t128_sb");
-        add_real<F32, 128, 2, false, 16, 16, 8>(*v, KFA_F32, "real2048_16168_t128x2_sb");
-        // N = 4000 (half 2000)
+        // N = 4000 (half 2000); sweep: t125x2_sb 4.41 / 4.46 TB/s, t250_sb 4.41 / 4.03
+        add_real<F32, 125, 2, false, 16, 5, 5, 5>(*v, KFA_F32, "real2000_16555_t125x2_sb");
         add_real<F32, 250, 1, false, 8, 2, 5, 5, 5>(*v, KFA_F32, "real2000_82555_t250_sb");
         add_real<F32, 100, 2, true, 4, 4, 5, 5, 5>(*v, KFA_F32, "real2000_44555_t100x2_db");
         add_real<F32, 100, 2, false, 4, 4, 5, 5, 5>(*v, KFA_F32, "real2000_44555_t100x2_sb");
         add_real<F32, 250, 1, false, 5, 5, 5, 4, 4>(*v, KFA_F32, "real2000_55544_t250_sb");
-        add_real<F32, 125, 2, false, 16, 5, 5, 5>(*v, KFA_F32, "real2000_16555_t125x2_sb");
         add_fused<F32, FK_R2C, 128, 1, 16, 16, 8>(*v, KFA_F32, "r2c2048_16168_t128_fused");
         add_fused<F32, FK_C2R, 128, 1, 8, 16, 16>(*v, KFA_F32, "c2r2048_81616_t128_fused");
         add_fused<F32, FK_R2C, 128, 2, 16, 16, 8>(*v, KFA_F32, "r2c2048_16168_t128x2_fused");
