@@ -106,7 +106,11 @@ const std::vector<FastVariant> &fast_registry()
         add_real<F32, 256, 1, true, 8, 8, 8, 4>(*v, KFA_F32, "real2048_8884_t256_db");
         add_real<F32, 128, 1, true, 16, 16, 8>(*v, KFA_F32, "real2048_16168_t128_db");
         add_real<F32, 128, 1, false, 8, 16, 16>(*v, KFA_F32, "real2048_81616_t128_sb");
-        // N = 4000 (half 2000); sweep: t125x2_sb 4.41 / 4.46 TB/s, t250_sb 4.41 / 4.03
+        // N = 4000 (half 2000); sweep, 67092 lines: t128x2_sb 4.88 / 4.82 TB/s (whole
+        // waves: lines of 125 or 250 threads straddle waves), t125x2_sb 4.13 / 4.34,
+        // t250_sb 4.35 / 4.01
+        add_real<F32, 128, 2, false, 16, 5, 5, 5>(*v, KFA_F32, "real2000_16555_t128x2_sb");
+        add_real<F32, 256, 1, false, 16, 5, 5, 5>(*v, KFA_F32, "real2000_16555_t256_sb");
         add_real<F32, 125, 2, false, 16, 5, 5, 5>(*v, KFA_F32, "real2000_16555_t125x2_sb");
         add_real<F32, 250, 1, false, 8, 2, 5, 5, 5>(*v, KFA_F32, "real2000_82555_t250_sb");
         add_real<F32, 100, 2, true, 4, 4, 5, 5, 5>(*v, KFA_F32, "real2000_44555_t100x2_db");
@@ -125,6 +129,9 @@ const std::vector<FastVariant> &fast_registry()
         add_fused<F32, FK_R2C, 200, 1, 8, 2, 5, 5, 5>(*v, KFA_F32, "r2c2000_82555_t200_fused");
         add_fused<F32, FK_C2R, 200, 1, 5, 5, 5, 2, 8>(*v, KFA_F32, "c2r2000_55528_t200_fused");
         // N = 1000 (half 500), the real axis of the 2-D workload
+        // (sweep, 268167 lines: t64x4 4.70 / 4.58 TB/s, t50x4 4.02 / 3.68 -- whole waves)
+        add_real<F32, 64, 4, false, 4, 5, 5, 5>(*v, KFA_F32, "real500_4555_t64x4_sb");
+        add_real<F32, 64, 2, false, 4, 5, 5, 5>(*v, KFA_F32, "real500_4555_t64x2_sb");
         add_real<F32, 50, 4, false, 4, 5, 5, 5>(*v, KFA_F32, "real500_4555_t50x4_sb");
         add_real<F32, 125, 2, false, 4, 5, 5, 5>(*v, KFA_F32, "real500_4555_t125x2_sb");
         add_fused<F32, FK_R2C, 50, 4, 4, 5, 5, 5>(*v, KFA_F32, "r2c500_4555_t50x4_fused");
@@ -140,14 +147,22 @@ const std::vector<FastVariant> &fast_registry()
         add_variant<F32, FK_C2C, 64, 4, false, false, 8, 8, 8>(*v, KFA_F32, "c2c512_888_t64x4_sb");
         add_variant<F32, FK_C2C, 16, 16, false, false, 16, 16>(*v, KFA_F32, "c2c256_1616_t16x16_sb");
         add_variant<F32, FK_C2C, 64, 4, false, false, 4, 4, 4, 4>(*v, KFA_F32, "c2c256_4444_t64x4_sb");
+        // whole waves per line first (sweeps: c2c1000 t64x4 5.06 TB/s, t125x2 4.67;
+        // c2c120 t16x16 5.21, t40x8 3.79; c2c500 t64x4 4.97, t125x2 4.87)
+        add_variant<F32, FK_C2C, 64, 4, false, false, 5, 5, 5, 8>(*v, KFA_F32, "c2c1000_5558_t64x4_sb");
+        add_variant<F32, FK_C2C, 128, 2, false, false, 5, 5, 5, 8>(*v, KFA_F32, "c2c1000_5558_t128x2_sb");
         add_variant<F32, FK_C2C, 125, 2, false, false, 5, 5, 5, 8>(*v, KFA_F32, "c2c1000_5558_t125x2_sb");
         add_variant<F32, FK_C2C, 100, 2, false, false, 5, 5, 5, 4, 2>(*v, KFA_F32, "c2c1000_55542_t100x2_sb");
         // N-D axis lengths (contiguous last axes; the non-last axes run as tiles)
-        add_variant<F32, FK_C2C, 40, 8, false, false, 8, 3, 5>(*v, KFA_F32, "c2c120_835_t40x8_sb");
+        add_variant<F32, FK_C2C, 16, 16, false, false, 8, 5, 3>(*v, KFA_F32, "c2c120_853_t16x16_sb");
         add_variant<F32, FK_C2C, 24, 8, false, false, 8, 5, 3>(*v, KFA_F32, "c2c120_853_t24x8_sb");
+        add_variant<F32, FK_C2C, 32, 8, false, false, 8, 5, 3>(*v, KFA_F32, "c2c120_853_t32x8_sb");
+        add_variant<F32, FK_C2C, 40, 8, false, false, 8, 3, 5>(*v, KFA_F32, "c2c120_835_t40x8_sb");
         add_variant<F32, FK_C2C, 16, 16, false, false, 8, 4, 3>(*v, KFA_F32, "c2c96_843_t16x16_sb");
         add_variant<F32, FK_C2C, 8, 32, false, false, 8, 8>(*v, KFA_F32, "c2c64_88_t8x32_sb");
         add_variant<F32, FK_C2C, 16, 16, false, false, 8, 16>(*v, KFA_F32, "c2c128_816_t16x16_sb");
+        add_variant<F32, FK_C2C, 64, 4, false, false, 5, 5, 5, 4>(*v, KFA_F32, "c2c500_5554_t64x4_sb");
+        add_variant<F32, FK_C2C, 128, 2, false, false, 5, 5, 5, 4>(*v, KFA_F32, "c2c500_5554_t128x2_sb");
         add_variant<F32, FK_C2C, 125, 2, false, false, 5, 5, 5, 4>(*v, KFA_F32, "c2c500_5554_t125x2_sb");
         add_variant<F32, FK_C2C, 192, 1, false, false, 3, 16, 8, 8>(*v, KFA_F32, "c2c3072_31688_t192_sb");
         add_variant<F32, FK_C2C, 256, 1, false, false, 16, 16, 16>(*v, KFA_F32, "c2c4096_161616_t256_sb");
