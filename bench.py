@@ -84,6 +84,7 @@ def _time(fn, stream, reps: int) -> float:
 
 def other_configs(lib, stream, dev) -> dict:
     """The reference's other workloads, each timed on its own (inputs resident in HBM)."""
+    import numpy as np
     import torch
 
     import kissfft_amd
@@ -136,6 +137,53 @@ def other_configs(lib, stream, dev) -> dict:
     rec("ndr_float_1024x1000_x32_fwd", t, nb, 32)
     t = _time(lambda: lib.fftndri_batch(ci, Xr, yr, 32, stream=stream), stream, 10)
     rec("ndr_float_1024x1000_x32_inv", t, nb, 32)
+    del xr, Xr, yr
+
+    # uneven batching: 64 batches of mixed lengths and kinds (generic prime 1009 included),
+    # batch counts log-uniform in [1, 512], as one kiss_fft_multi launch replayed from a
+    # hipGraph, against the same batches as one batched call each on the same stream
+    rng = np.random.default_rng(11)
+    menu = [(kissfft_amd.MultiBatch.C2C_FWD, 256), (kissfft_amd.MultiBatch.C2C_FWD, 1000),
+            (kissfft_amd.MultiBatch.C2C_FWD, 1009), (kissfft_amd.MultiBatch.C2C_FWD, 1200),
+            (kissfft_amd.MultiBatch.C2C_INV, 3072), (kissfft_amd.MultiBatch.R2C, 4096),
+            (kissfft_amd.MultiBatch.C2R, 4000), (kissfft_amd.MultiBatch.C2C_FWD, 120)]
+    items, calls, mbytes, mtrans = [], [], 0, 0
+    for i in range(64):
+        kind, n = menu[i % len(menu)]
+        cnt = int(np.exp(rng.uniform(0, np.log(512))))
+        nin = (n * cnt * 2) if kind in (0, 1) else (n * cnt if kind == 2 else (n // 2 + 1) * cnt * 2)
+        nout = (n * cnt * 2) if kind in (0, 1) else ((n // 2 + 1) * cnt * 2 if kind == 2 else n * cnt)
+        with torch.cuda.stream(stream):
+            fi = torch.rand(nin, device=dev) * 2 - 1
+            fo = torch.empty(nout, device=dev)
+        items.append((kind, n, fi, fo, cnt))
+        mbytes += 4 * (nin + nout)
+        mtrans += cnt
+        if kind in (0, 1):
+            cf1 = lib.fft_alloc(n, kind == 1)
+            calls.append(lambda c=cf1, a=fi, b=fo, k=cnt, m=n: lib.fft_batch(c, a, b, k, m, m, stream=stream))
+        elif kind == 2:
+            cf1 = lib.fftr_alloc(n)
+            calls.append(lambda c=cf1, a=fi, b=fo, k=cnt, m=n: lib.fftr_batch(c, a, b, k, m, m // 2 + 1,
+                                                                              stream=stream))
+        else:
+            cf1 = lib.fftr_alloc(n, True)
+            calls.append(lambda c=cf1, a=fi, b=fo, k=cnt, m=n: lib.fftri_batch(c, a, b, k, m // 2 + 1, m,
+                                                                               stream=stream))
+    torch.cuda.synchronize()
+    mb = kissfft_amd.MultiBatch(lib, items)
+    mb.capture(stream)
+    t_multi = _time(lambda: mb.replay(stream), stream, 20)
+
+    def per_batch():
+        for c in calls:
+            c()
+    t_calls = _time(per_batch, stream, 20)
+    rec("multi_uneven_64_batches", t_multi, mbytes, mtrans,
+        per_batch_calls_us=round(t_calls * 1e6, 1), speedup_vs_per_batch_calls=round(t_calls / t_multi, 2),
+        kinds_lengths="c2c 256/1000/1009/1200/120, c2c inverse 3072, r2c 4096, c2r 4000",
+        launch=mb.info())
+    mb.close()
     return out
 
 

@@ -5,16 +5,25 @@
  * reference analogues are kiss_fftnd's loop over many columns (kiss_fftnd.c:176-177) and
  * the SIMD build's transforms in lockstep (README.simd:12-20).
  *
- * A multi-batch fixes a list of items -- kind, length, device buffers, count -- and
- * executes all of them with a single persistent kernel launch: mixed lengths (any
- * factorisation, generic prime radices included) and mixed complex / real kinds together.
- * The launch can be captured into a hipGraph once and replayed.  Results equal the
- * per-item batched calls of the fixed-point builds bit for bit, and the float builds'
- * within the float tolerance (the float batched calls may use other radix schedules).
- *
- * Items whose length exceeds one LDS line (float 8192 complex / 16384 real, Q15 16384 /
- * 32768) run as their own multi-pass launches after the persistent one; a multi-batch that
- * holds such items cannot be captured.  kiss_fft_cleanup() invalidates every multi-batch.
+ * A multi-batch fixes a list of items -- kind, length, device buffers, count -- and runs
+ * all of them with one call (or one replay of a captured hipGraph).  At creation each item
+ * is assigned, once, to a launch:
+ *   - items whose (kind, length) has a compiled row kernel are grouped by (kind, length)
+ *     into one launch of that kernel (up to 64 items per launch, one segment each);
+ *   - float lengths with a prime factor above 16 (e.g. 1009) run Bluestein's kernel,
+ *     grouped by length the same way;
+ *   - every other item of at most one LDS line (generic prime radices, fixed-point lengths
+ *     without a compiled kernel) joins a single persistent launch that walks a table of
+ *     all of them, any mix of lengths and kinds together;
+ *   - items longer than one LDS line (float 8192 complex / 16384 real, Q15 16384 / 32768)
+ *     run as their own multi-pass launches; a multi-batch holding such items cannot be
+ *     captured (they allocate scratch).
+ * The launches are spread over the multi-batch's own streams and joined on the caller's
+ * stream, so independent groups overlap; capture records the same fork / join.
+ * KISSFFT_AMD_MULTI_PERSISTENT=1 at creation puts every single-pass item into the
+ * persistent launch.  Results equal the per-item batched calls of the fixed-point builds
+ * bit for bit, and the float builds' within the float tolerance.  Items must not share
+ * output memory.  kiss_fft_cleanup() invalidates every multi-batch.
  */
 #ifndef KISS_FFT_MULTI_H
 #define KISS_FFT_MULTI_H
@@ -57,6 +66,10 @@ int KISS_FFT_API kiss_fft_multi_replay(kiss_fft_multi m, void *stream);
 
 /* Tiles of the persistent launch, its grid, and the items it covers. */
 int KISS_FFT_API kiss_fft_multi_info(kiss_fft_multi m, long long *tiles, int *grid, int *persistent_items);
+
+/* Compiled-kernel launches and the items they cover, Bluestein items, multi-pass items. */
+int KISS_FFT_API kiss_fft_multi_layout(kiss_fft_multi m, int *groups, int *group_items, int *bluestein_items,
+                                       int *multipass_items);
 
 void KISS_FFT_API kiss_fft_multi_free(kiss_fft_multi m);
 

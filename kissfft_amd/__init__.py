@@ -151,6 +151,8 @@ class Library:
             "kiss_fft_multi_capture": ([vp, vp], ci),
             "kiss_fft_multi_replay": ([vp, vp], ci),
             "kiss_fft_multi_info": ([vp, ctypes.POINTER(ll), ctypes.POINTER(ci), ctypes.POINTER(ci)], ci),
+            "kiss_fft_multi_layout": ([vp, ctypes.POINTER(ci), ctypes.POINTER(ci), ctypes.POINTER(ci),
+                                       ctypes.POINTER(ci)], ci),
             "kiss_fft_multi_free": ([vp], None),
             "kiss_fastfir_alloc_cpx": ([vp, ctypes.c_size_t, szp, vp, szp], vp),
             "kiss_fastfir_alloc_real": ([vp, ctypes.c_size_t, szp, vp, szp], vp),
@@ -333,7 +335,7 @@ class MultiItem(ctypes.Structure):
 
 
 class MultiBatch:
-    """A kiss_fft_multi: a fixed set of batches of mixed lengths and kinds, one launch.
+    """A kiss_fft_multi: a fixed set of batches of mixed lengths and kinds, run with one call.
 
     Kinds as for the server (C2C_FWD, C2C_INV, R2C, C2R).  The buffers must stay alive as
     long as the multi-batch is used."""
@@ -361,7 +363,11 @@ class MultiBatch:
     def info(self) -> dict:
         t, g, p = ctypes.c_longlong(), ctypes.c_int(), ctypes.c_int()
         self.lib.L.kiss_fft_multi_info(self.ptr, ctypes.byref(t), ctypes.byref(g), ctypes.byref(p))
-        return {"tiles": t.value, "grid": g.value, "persistent_items": p.value}
+        gr, gi, bl, mp = (ctypes.c_int() for _ in range(4))
+        self.lib.L.kiss_fft_multi_layout(self.ptr, ctypes.byref(gr), ctypes.byref(gi), ctypes.byref(bl),
+                                         ctypes.byref(mp))
+        return {"tiles": t.value, "grid": g.value, "persistent_items": p.value, "groups": gr.value,
+                "group_items": gi.value, "bluestein_items": bl.value, "multipass_items": mp.value}
 
     def close(self) -> None:
         if getattr(self, "ptr", None):

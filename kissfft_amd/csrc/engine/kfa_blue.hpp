@@ -26,6 +26,8 @@ struct BlueArgs {
     int n;
     long long nlines;
     long long in_hi, in_el, out_hi, out_el;
+    int nseg;                   // 0: one buffer (in / out); else segments as in FastArgs
+    FastSeg seg[KFA_MAX_SEG];
 };
 
 template <int T, int LPB, class RS>
@@ -46,8 +48,10 @@ __global__ void __launch_bounds__(T *LPB) k_blue(BlueArgs a)
     const St *chirp = reinterpret_cast<const St *>(a.chirp);
     const St *bf = reinterpret_cast<const St *>(a.bfft);
 
+    const St *src;
+    St *dst;
+    line_base<St>(a, ok ? gl : 0, src, dst);
     {
-        const St *src = reinterpret_cast<const St *>(a.in) + (ok ? gl : 0) * a.in_hi;
         C v[IT];
 #pragma unroll
         for (int k = 0; k < IT; ++k) {
@@ -71,7 +75,6 @@ __global__ void __launch_bounds__(T *LPB) k_blue(BlueArgs a)
                                   reinterpret_cast<const St *>(a.stw_i), t);
     if (!ok)
         return;
-    St *dst = reinterpret_cast<St *>(a.out) + gl * a.out_hi;
 #pragma unroll
     for (int k = 0; k < IT; ++k) {
         const int i = t + k * T;

@@ -761,6 +761,35 @@ static int blue_choice(const DevPlan &P, const BlueChoice **out)
     return 0;
 }
 
+// Whether float plan P runs Bluestein's kernel; its tables are built now rather than at
+// the first launch (the multi-batch decides at creation, before any capture).
+int blue_prepare(const DevPlan &P, bool *applies)
+{
+    const BlueChoice *bc = nullptr;
+    const int rc = blue_choice(P, &bc);
+    *applies = rc == 0 && bc->v != nullptr;
+    return rc;
+}
+
+// Bluestein launch for float plan P with b's buffers / segments / distances filled in by
+// the caller (the multi-batch's grouped items); the tables come from P's cached choice.
+int blue_launch(const DevPlan &P, BlueArgs &b, hipStream_t s)
+{
+    const BlueChoice *bc = nullptr;
+    int rc = blue_choice(P, &bc);
+    if (rc)
+        return rc;
+    if (!bc->v)
+        return fail(-2, "no Bluestein kernel for n=%d", P.n);
+    b.stw_f = bc->stw_f;
+    b.stw_i = bc->stw_i;
+    b.chirp = bc->chirp;
+    b.bfft = bc->bfft;
+    b.n = P.n;
+    ++g_launches;
+    return bc->v->launch(b, s) ? fail(-3, "Bluestein kernel launch failed") : 0;
+}
+
 static void blue_release()
 {
     std::lock_guard<std::mutex> g(g_blue_mu);

@@ -113,9 +113,11 @@ __device__ __forceinline__ void hbm_st(St *p, St v)
     *p = v;
 }
 
-// Base pointers of global line gl: its segment's buffers and its index inside them.
-template <class S>
-__device__ __forceinline__ void line_base(const FastArgs &a, long long gl, const S *&in, S *&out)
+// Base pointers of global line gl: one buffer (nseg == 0) or the segment holding gl, at
+// gl's index inside it.  Any argument block with in / out / in_hi / out_hi / nseg / seg
+// (FastArgs, BlueArgs).
+template <class S, class Args>
+__device__ __forceinline__ void line_base(const Args &a, long long gl, const S *&in, S *&out)
 {
     if (a.nseg <= 0) {
         in = reinterpret_cast<const S *>(a.in) + gl * a.in_hi;

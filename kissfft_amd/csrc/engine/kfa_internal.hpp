@@ -91,6 +91,12 @@ std::vector<unsigned char> float_twiddles(long long n, int inverse, long long co
 int make_stage_table_raw(const unsigned char *h_tw, int type, long long n, const std::vector<int> &rad, int pad,
                          int fmt, void **d_out);
 
+// whether a float plan runs Bluestein's kernel (its tables built on return)
+int blue_prepare(const DevPlan &P, bool *applies);
+// one Bluestein launch for plan P; b carries buffers, segments, counts and distances
+struct BlueArgs;
+int blue_launch(const DevPlan &P, BlueArgs &b, hipStream_t s);
+
 // releases the fast-convolution device plans (kfa_conv.hip)
 void conv_release();
 

@@ -1,23 +1,35 @@
-// kfa_multi.hip -- several batches of different lengths and kinds in one persistent launch.
+// kfa_multi.hip -- several batches of different lengths and kinds in one launch sequence.
 //
 // The batches (items) are fixed when the multi-batch is created: kind, length, device
-// buffers and count.  Creation builds (or finds) each length's device plan and uploads one
-// table describing every item: its stage list, twiddles, buffers and its range in a global
-// list of tiles.  Execution is a single launch of k_multi (kfa_lines.hpp): a grid sized to
-// the device walks the tile list, each workgroup transforming whole tiles with the
-// general LDS-resident line code, so any mix of radices -- including generic primes -- and
-// of complex / real kinds runs together, bit-exact in the fixed-point builds.  Items longer
-// than one LDS line run as their own multi-pass launches after it.  The launch uses no
-// counters or memsets and allocates nothing, so it can be captured into a hipGraph once and
-// replayed.
+// buffers and count.  Creation builds (or finds) each length's device plan and decides, once,
+// how each item runs:
+//   * items whose (kind, length) has a compiled row kernel are grouped by (kind, length):
+//     one launch of that kernel per group, its segment table mapping lines to the items'
+//     buffers (as the request server does), the variant chosen by the engine's tuner;
+//   * float lengths with a prime factor above 16 run the Bluestein kernel, grouped by
+//     (kind, length) the same way;
+//   * every other single-pass item (generic radices, fixed-point lengths without a compiled
+//     kernel) joins one persistent launch of k_multi (kfa_lines.hpp): a grid sized to the
+//     device walks a global list of tiles with the general LDS-resident line code, any mix of
+//     radices and kinds together, bit-exact in the fixed-point builds;
+//   * items longer than one LDS line run as their own multi-pass launches (these allocate
+//     scratch, so a multi-batch holding them cannot be captured).
+// KISSFFT_AMD_MULTI_PERSISTENT=1 puts every single-pass item into the persistent launch.
+// Execution forks the launches over the multi-batch's helper streams (round robin, largest
+// first) and joins them on the caller's stream, so independent groups overlap; capture
+// records the same fork / join into one hipGraph.  Nothing is allocated and no memset is
+// issued on the capturable path.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <vector>
 
 #include "../kfa_engine.h"
+#include "kfa_blue.hpp"
+#include "kfa_fast.hpp"
 #include "kfa_internal.hpp"
 
 namespace kfa {
@@ -32,7 +44,7 @@ struct Multi {
     long long ntiles = 0;
     size_t lds = 0;
     int grid = 0;
-    struct Long {                        // multi-pass items, launched on their own
+    struct Long {                        // items launched through the engine's batched path
         int kind;
         std::shared_ptr<DevPlan> P;
         std::shared_ptr<RealTables> R;
@@ -41,10 +53,26 @@ struct Multi {
         long long howmany;
         int n;
     };
-    std::vector<Long> longs;
+    std::vector<Long> longs;             // multi-pass (allocate scratch)
+    struct Group {                       // one segmented launch of a compiled row kernel
+        const FastVariant *v;
+        FastArgs a;
+        size_t bytes;
+    };
+    std::vector<Group> groups;
+    int group_items = 0;
+    struct BlueGroup {                   // one segmented Bluestein launch (float primes > 16)
+        std::shared_ptr<DevPlan> P;
+        BlueArgs a;
+    };
+    std::vector<BlueGroup> blue;
+    int blue_items = 0;
+    std::vector<hipStream_t> helpers;    // fork / join streams (default priority)
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
 };
+
+static constexpr int kMultiHelpers = 3;
 
 template <class A>
 static int launch_multi(Multi &M, hipStream_t s)
@@ -60,26 +88,64 @@ static int launch_multi(Multi &M, hipStream_t s)
     return e == hipSuccess ? 0 : fail(-3, "multi-batch launch failed: %s", hipGetErrorString(e));
 }
 
+static int run_long(const Multi::Long &L, hipStream_t s)
+{
+    const long long nc = L.P->n;
+    if (L.kind == KFA_SERVE_R2C)
+        return run_real_rows(*L.P, *L.R, 0, L.in, nc, L.out, nc + 1, L.howmany, s);
+    if (L.kind == KFA_SERVE_C2R)
+        return run_real_rows(*L.P, *L.R, 1, L.in, nc + 1, L.out, nc, L.howmany, s);
+    return run_c2c(*L.P, L.in, 1, nc, L.out, 1, nc, L.howmany, s);
+}
+
+// Every launch of the multi-batch, forked over s and the helper streams and joined on s.
 static int run_multi(Multi &M, hipStream_t s)
 {
+    std::vector<hipStream_t> st{s};
+    st.insert(st.end(), M.helpers.begin(), M.helpers.end());
+    std::vector<hipEvent_t> ev(st.size(), nullptr);
+    for (auto &e : ev)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            for (auto &x : ev)
+                if (x)
+                    (void)hipEventDestroy(x);
+            return fail(-3, "multi-batch: event creation failed");
+        }
     int rc = 0;
-    if (M.ntiles > 0) {
+    auto check = [&](hipError_t e) {
+        if (e != hipSuccess && rc == 0)
+            rc = fail(-3, "multi-batch fork / join: %s", hipGetErrorString(e));
+    };
+    check(hipEventRecord(ev[0], s));
+    for (size_t i = 1; i < st.size(); ++i)
+        check(hipStreamWaitEvent(st[i], ev[0], 0));
+    size_t next = 0;
+    auto pick = [&]() { return st[next++ % st.size()]; };
+    // groups are sorted largest first at creation
+    for (size_t i = 0; i < M.groups.size() && rc == 0; ++i) {
+        ++g_launches;
+        if (M.groups[i].v->launch(M.groups[i].a, pick()))
+            rc = fail(-3, "fast kernel launch failed");
+    }
+    for (size_t i = 0; i < M.blue.size() && rc == 0; ++i)
+        rc = blue_launch(*M.blue[i].P, M.blue[i].a, pick());
+    if (rc == 0 && M.ntiles > 0) {
+        hipStream_t sp = pick();
         switch (M.type) {
-        case KFA_F32: rc = launch_multi<F32>(M, s); break;
-        case KFA_Q15: rc = launch_multi<Q15>(M, s); break;
-        default: rc = launch_multi<Q31>(M, s); break;
+        case KFA_F32: rc = launch_multi<F32>(M, sp); break;
+        case KFA_Q15: rc = launch_multi<Q15>(M, sp); break;
+        default: rc = launch_multi<Q31>(M, sp); break;
         }
     }
-    for (size_t i = 0; i < M.longs.size() && rc == 0; ++i) {
-        const Multi::Long &L = M.longs[i];
-        const long long nc = L.P->n;
-        if (L.kind == KFA_SERVE_R2C)
-            rc = run_real_rows(*L.P, *L.R, 0, L.in, nc, L.out, nc + 1, L.howmany, s);
-        else if (L.kind == KFA_SERVE_C2R)
-            rc = run_real_rows(*L.P, *L.R, 1, L.in, nc + 1, L.out, nc, L.howmany, s);
-        else
-            rc = run_c2c(*L.P, L.in, 1, nc, L.out, 1, nc, L.howmany, s);
+    for (size_t i = 0; i < M.longs.size() && rc == 0; ++i)
+        rc = run_long(M.longs[i], s);
+    // join (also after a failure: the helper streams must leave a capture)
+    for (size_t i = 1; i < st.size(); ++i) {
+        check(hipEventRecord(ev[i], st[i]));
+        check(hipStreamWaitEvent(s, ev[i], 0));
     }
+    for (auto &e : ev)
+        (void)hipEventDestroy(e);
     return rc;
 }
 
@@ -102,6 +168,26 @@ void *kfa_multi_create(int type, const kfa_multi_item *items, int nitems)
     M->type = type;
     M->dev = dev;
     const size_t cb = cpx_bytes(type);
+    const bool all_persistent = env_flag("KISSFFT_AMD_MULTI_PERSISTENT", false);
+    // decisions (variant tuning) happen here, on a stream of their own, never during capture
+    hipStream_t s0 = nullptr;
+    if (hipStreamCreateWithFlags(&s0, hipStreamNonBlocking) != hipSuccess) {
+        fail(-3, "multi-batch: stream creation failed");
+        return nullptr;
+    }
+    struct StreamGuard {
+        hipStream_t s;
+        ~StreamGuard()
+        {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
+    } guard{s0};
+    std::map<std::pair<int, int>, std::vector<int>> by_key;  // (kind, nfft) -> compiled items
+    std::map<std::pair<int, int>, std::vector<int>> by_blue; // (kind, nfft) -> Bluestein items
+    std::vector<const FastChoice *> choice(nitems, nullptr);
+    std::vector<std::shared_ptr<DevPlan>> iplan(nitems);
+    std::vector<std::shared_ptr<RealTables>> irtab(nitems);
     long long tiles = 0;
     for (int i = 0; i < nitems; ++i) {
         const kfa_multi_item &it = items[i];
@@ -126,10 +212,30 @@ void *kfa_multi_create(int type, const kfa_multi_item *items, int nitems)
         M->plans.push_back(P);
         if (R)
             M->rtabs.push_back(R);
+        iplan[i] = P;
+        irtab[i] = R;
         const long long nc = P->n;
         if (!P->single) {
             M->longs.push_back(Multi::Long{it.kind, P, R, it.in, it.out, it.howmany, it.nfft});
             continue;
+        }
+        if (!all_persistent) {
+            const int fk = it.kind == KFA_SERVE_R2C ? FK_R2C : it.kind == KFA_SERVE_C2R ? FK_C2R : FK_C2C;
+            const FastChoice *fc = nullptr;
+            if ((rc = choose_fast(*P, fk, s0, &fc, it.howmany)))
+                return nullptr;
+            if (fc->v) {
+                choice[i] = fc;
+                by_key[{it.kind, it.nfft}].push_back(i);
+                continue;
+            }
+            bool blue = false;
+            if (type == KFA_F32 && !real && (rc = blue_prepare(*P, &blue)))
+                return nullptr;
+            if (blue) {
+                by_blue[{it.kind, it.nfft}].push_back(i);
+                continue;
+            }
         }
         // lines per tile as in the line kernel's row mode
         const size_t per_line = 2 * (size_t)nc * cb;
@@ -159,6 +265,75 @@ void *kfa_multi_create(int type, const kfa_multi_item *items, int nitems)
         M->h.push_back(mi);
     }
     M->ntiles = tiles;
+    // one segmented launch per (kind, length) and at most KFA_MAX_SEG items
+    for (auto &kv : by_key) {
+        const std::vector<int> &idx = kv.second;
+        for (size_t g0 = 0; g0 < idx.size(); g0 += KFA_MAX_SEG) {
+            const size_t g1 = std::min(idx.size(), g0 + KFA_MAX_SEG);
+            const kfa_multi_item &f = items[idx[g0]];
+            const bool real = f.kind == KFA_SERVE_R2C || f.kind == KFA_SERVE_C2R;
+            const long long nc = iplan[idx[g0]]->n;
+            Multi::Group G{};
+            G.v = choice[idx[g0]]->v;
+            G.a.stw = choice[idx[g0]]->d_stw;
+            G.a.rtw = real ? irtab[idx[g0]]->d_stw : nullptr;
+            G.a.in_hi = f.kind == KFA_SERVE_C2R ? nc + 1 : nc;
+            G.a.out_hi = f.kind == KFA_SERVE_R2C ? nc + 1 : nc;
+            G.a.in_el = G.a.out_el = 1;
+            G.a.nseg = (int)(g1 - g0);
+            long long line = 0;
+            for (size_t j = g0; j < g1; ++j) {
+                const kfa_multi_item &it = items[idx[j]];
+                G.a.seg[j - g0].first = line;
+                G.a.seg[j - g0].in = it.in;
+                G.a.seg[j - g0].out = it.out;
+                line += it.howmany;
+            }
+            G.a.in = items[idx[g0]].in;
+            G.a.out = items[idx[g0]].out;
+            G.a.nlines = line;
+            G.bytes = (size_t)line * cb * (size_t)(G.a.in_hi + G.a.out_hi);
+            M->groups.push_back(G);
+            M->group_items += (int)(g1 - g0);
+        }
+    }
+    for (auto &kv : by_blue) {
+        const std::vector<int> &idx = kv.second;
+        for (size_t g0 = 0; g0 < idx.size(); g0 += KFA_MAX_SEG) {
+            const size_t g1 = std::min(idx.size(), g0 + KFA_MAX_SEG);
+            Multi::BlueGroup G{};
+            G.P = iplan[idx[g0]];
+            const long long n = G.P->n;
+            G.a.in_hi = G.a.out_hi = n;
+            G.a.in_el = G.a.out_el = 1;
+            G.a.nseg = (int)(g1 - g0);
+            long long line = 0;
+            for (size_t j = g0; j < g1; ++j) {
+                const kfa_multi_item &it = items[idx[j]];
+                G.a.seg[j - g0].first = line;
+                G.a.seg[j - g0].in = it.in;
+                G.a.seg[j - g0].out = it.out;
+                line += it.howmany;
+            }
+            G.a.in = items[idx[g0]].in;
+            G.a.out = items[idx[g0]].out;
+            G.a.nlines = line;
+            M->blue.push_back(G);
+            M->blue_items += (int)(g1 - g0);
+        }
+    }
+    std::stable_sort(M->groups.begin(), M->groups.end(),
+                     [](const Multi::Group &x, const Multi::Group &y) { return x.bytes > y.bytes; });
+    for (int i = 0; i < kMultiHelpers; ++i) {
+        hipStream_t hs = nullptr;
+        if (hipStreamCreateWithFlags(&hs, hipStreamNonBlocking) != hipSuccess) {
+            fail(-3, "multi-batch: stream creation failed");
+            for (hipStream_t x : M->helpers)
+                (void)hipStreamDestroy(x);
+            return nullptr;
+        }
+        M->helpers.push_back(hs);
+    }
     if (!M->h.empty()) {
         const size_t bytes = sizeof(MultiItem) * M->h.size();
         hipError_t e = hipMalloc(&M->d, bytes);
@@ -171,6 +346,8 @@ void *kfa_multi_create(int type, const kfa_multi_item *items, int nitems)
             fail(-3, "multi-batch setup: %s", hipGetErrorString(e));
             if (M->d)
                 (void)hipFree(M->d);
+            for (hipStream_t x : M->helpers)
+                (void)hipStreamDestroy(x);
             return nullptr;
         }
         // enough resident workgroups per CU to overlap memory and arithmetic
@@ -252,6 +429,22 @@ int kfa_multi_info(void *m, long long *tiles, int *grid, int *persistent_items)
     return 0;
 }
 
+int kfa_multi_layout(void *m, int *groups, int *group_items, int *bluestein_items, int *multipass_items)
+{
+    Multi *M = static_cast<Multi *>(m);
+    if (!M)
+        return -2;
+    if (groups)
+        *groups = (int)M->groups.size();
+    if (group_items)
+        *group_items = M->group_items;
+    if (bluestein_items)
+        *bluestein_items = M->blue_items;
+    if (multipass_items)
+        *multipass_items = (int)M->longs.size();
+    return 0;
+}
+
 void kfa_multi_destroy(void *m)
 {
     Multi *M = static_cast<Multi *>(m);
@@ -263,6 +456,8 @@ void kfa_multi_destroy(void *m)
         (void)hipGraphDestroy(M->graph);
     if (M->d)
         (void)hipFree(M->d);
+    for (hipStream_t x : M->helpers)
+        (void)hipStreamDestroy(x);
     delete M;
 }
 

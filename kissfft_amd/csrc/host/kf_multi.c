@@ -99,6 +99,12 @@ int kiss_fft_multi_info(kiss_fft_multi m, long long *tiles, int *grid, int *pers
     return m ? kfa_multi_info(m->engine, tiles, grid, persistent_items) : -2;
 }
 
+int kiss_fft_multi_layout(kiss_fft_multi m, int *groups, int *group_items, int *bluestein_items,
+                          int *multipass_items)
+{
+    return m ? kfa_multi_layout(m->engine, groups, group_items, bluestein_items, multipass_items) : -2;
+}
+
 void kiss_fft_multi_free(kiss_fft_multi m)
 {
     if (!m)

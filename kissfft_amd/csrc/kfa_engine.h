@@ -104,7 +104,7 @@ typedef struct {
 } kfa_stats;
 KFA_HIDDEN void kfa_get_stats(kfa_stats *s);
 
-/* Multi-batch: several batches of different lengths / kinds in one persistent launch. */
+/* Multi-batch: several batches of different lengths / kinds in one launch sequence. */
 typedef struct {
     int kind;              /* KFA_SERVE_* */
     int nfft;
@@ -120,6 +120,8 @@ KFA_HIDDEN int kfa_multi_exec(void *m, void *stream, int sync);
 KFA_HIDDEN int kfa_multi_capture(void *m, void *stream);
 KFA_HIDDEN int kfa_multi_replay(void *m, void *stream, int sync);
 KFA_HIDDEN int kfa_multi_info(void *m, long long *tiles, int *grid, int *persistent_items);
+KFA_HIDDEN int kfa_multi_layout(void *m, int *groups, int *group_items, int *bluestein_items,
+                                int *multipass_items);
 KFA_HIDDEN void kfa_multi_destroy(void *m);
 
 /* Fast FIR by overlap-scrap: nblocks windows in[k*ngood .. +nfft) -> out[k*ngood .. +ngood).

@@ -1,5 +1,8 @@
 """GPU tests of the multi-batch (include/kiss_fft_multi.h): mixed lengths and kinds in one
-persistent launch, and its hipGraph capture / replay."""
+call -- compiled-kernel groups, Bluestein items, the persistent general launch, multi-pass
+items -- and its hipGraph capture / replay.  Every test runs twice: with the default
+layout and with every single-pass item in the persistent launch
+(KISSFFT_AMD_MULTI_PERSISTENT=1)."""
 import sys
 
 import numpy as np
@@ -11,6 +14,12 @@ from kissfft_amd import workloads as W
 from oracle import Oracle
 
 pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(params=["grouped", "persistent"])
+def layout(request, monkeypatch):
+    monkeypatch.setenv("KISSFFT_AMD_MULTI_PERSISTENT", "1" if request.param == "persistent" else "0")
+    return request.param
 
 C2C, C2C_INV, R2C, C2R = 0, 1, 2, 3
 
@@ -49,16 +58,24 @@ FLOAT_ITEMS = [(C2C, 256, 100), (C2C, 1009, 7), (C2C_INV, 1200, 33), (R2C, 4096,
                (C2C, 65536, 2), (R2C, 40000, 2)]
 
 
-def test_float_mix_one_launch():
+def test_float_mix(layout):
     import torch
     lib = kissfft_amd.load("float")
     dev = torch.device("cuda:0")
     bufs = [make(torch, dev, k, n, b, 200 + i, "float") for i, (k, n, b) in enumerate(FLOAT_ITEMS)]
     m = lib.multi([(k, n, x, y, b) for (k, n, b), (x, y) in zip(FLOAT_ITEMS, bufs)])
     info = m.info()
-    say("multi info", info)
-    assert info["persistent_items"] == len(FLOAT_ITEMS) - 2  # 65536 and 40000 are multi-pass
-    assert info["grid"] > 0 and info["tiles"] > 0
+    say("multi info", layout, info)
+    assert info["multipass_items"] == 2  # 65536 and 40000
+    assert info["persistent_items"] + info["group_items"] + info["bluestein_items"] == len(FLOAT_ITEMS) - 2
+    if layout == "persistent":
+        assert info["persistent_items"] == len(FLOAT_ITEMS) - 2 and info["groups"] == 0
+        assert info["grid"] > 0 and info["tiles"] > 0
+    else:
+        # compiled kernels for 256, 1200 inverse, r2c 4096, c2r 4000; Bluestein for 1009;
+        # 60, 1001, r2c 6, 1, c2r 2 and 8192 in the persistent launch
+        assert info["group_items"] == 4 and info["groups"] == 4 and info["bluestein_items"] == 1
+        assert info["persistent_items"] == 6
     m.exec()
     for (k, n, b), (x, y) in zip(FLOAT_ITEMS, bufs):
         say("item", k, n, b)
@@ -78,7 +95,7 @@ Q15_ITEMS = [(C2C, 512, 64), (C2C, 1200, 5), (R2C, 4096, 3), (C2R, 500, 4), (C2C
              (C2C, 12, 17), (C2C, 65536, 1)]
 
 
-def test_q15_mix_bit_exact():
+def test_q15_mix_bit_exact(layout):
     import torch
     lib = kissfft_amd.load("int16_t")
     O = Oracle("int16_t")
@@ -95,7 +112,7 @@ def test_q15_mix_bit_exact():
     m.close()
 
 
-def test_graph_capture_and_replays():
+def test_graph_capture_and_replays(layout):
     # replays interleaved with eager work on the same stream; every replay is checked
     # against an eager execution of the same multi-batch on the same inputs
     import torch
@@ -125,4 +142,30 @@ def test_graph_capture_and_replays():
             got = y.cpu().numpy()
             got = got if k == C2R else cplx(got)
             assert rel_l2(got, fp64_ref(k, n, x.cpu().numpy())) <= ABS_FLOAT_BOUND, (rep, k, n)
+    m.close()
+
+
+def test_many_small_batches_grouped():
+    """64 batches of 8 (kind, length) keys, counts 1..40: one launch per key, same results
+    as the persistent layout (float tolerance) and as the per-batch calls."""
+    import torch
+    lib = kissfft_amd.load("float")
+    dev = torch.device("cuda:0")
+    menu = [(C2C, 256), (C2C, 1000), (C2C, 1009), (C2C_INV, 1200), (R2C, 4096), (C2R, 4000), (C2C, 120),
+            (C2C_INV, 3072)]
+    rng = np.random.default_rng(12)
+    items = [(menu[i % 8][0], menu[i % 8][1], int(rng.integers(1, 41))) for i in range(64)]
+    bufs = [make(torch, dev, k, n, b, 500 + i, "float") for i, (k, n, b) in enumerate(items)]
+    m = lib.multi([(k, n, x, y, b) for (k, n, b), (x, y) in zip(items, bufs)])
+    info = m.info()
+    say("many small", info)
+    assert info["groups"] == 7 and info["group_items"] == 56 and info["bluestein_items"] == 8
+    st = torch.cuda.Stream()
+    m.capture(st)
+    m.replay(st)
+    st.synchronize()
+    for (k, n, b), (x, y) in zip(items, bufs):
+        got = y.cpu().numpy()
+        got = got if k == C2R else cplx(got)
+        assert rel_l2(got, fp64_ref(k, n, x.cpu().numpy())) <= ABS_FLOAT_BOUND, (k, n, b)
     m.close()
