@@ -4,7 +4,7 @@
 // array per axis.  When a plane of the last two axes (A x B complex values, contiguous in
 // HBM) fits in one CU's LDS, a workgroup can do both axes of a plane between one load and
 // one store: rows of B points, then columns of A points, all in LDS.  An array of d axes
-// then costs d-1 HBM passes instead of d (the 3-D 64 x 96 x 120 case: 2 instead of 3).
+// then costs d-1 HBM passes instead of d (64 x 64 x 64: 2 instead of 3).
 //
 // Layout in LDS: row r at r*RSX (RSX = B+1 when B is even, so that a column's elements
 // fall in different banks).  Row stages use TB threads per row and unit element stride;

@@ -214,7 +214,10 @@ const std::vector<PlaneVariant> &plane_registry()
 {
     static std::vector<PlaneVariant> *reg = [] {
         auto *v = new std::vector<PlaneVariant>();
-        add_plane<8, 8, 960, Radices<2, 8, 4, 3>, Radices<2, 8, 5, 3>>(*v, "plane96x120_t8x8");
+        // Planes that leave room for several workgroups per CU.  A 96 x 120 plane (92 KB,
+        // one workgroup per CU, 12 barriers per plane) measured slower than its two axis
+        // passes on the 64x96x120 x16 array, which fits in the Infinity Cache (108.7 against
+        // 101.1 us), and 3 % faster on 96x120 x1024: not registered.
         add_plane<8, 8, 512, Radices<2, 8, 8>, Radices<2, 8, 8>>(*v, "plane64x64_t8x8");
         add_plane<8, 8, 1024, Radices<2, 16, 8>, Radices<2, 16, 8>>(*v, "plane128x128_t8x8");
         add_plane<4, 4, 128, Radices<2, 8, 4>, Radices<2, 8, 4>>(*v, "plane32x32_t4x4");
