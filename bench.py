@@ -300,6 +300,9 @@ def main() -> int:
     ap.add_argument("--pmc-run", action="store_true",
                     help="for rocprofv3 --pmc: one plain copy of a known size (the counter scale), "
                          "then one forward and one inverse N=4096 launch, and exit")
+    ap.add_argument("--pmc-long-real", action="store_true",
+                    help="for rocprofv3: 256 real signals of 2^20 points, forward and inverse three "
+                         "times each (the fused four-step real passes), and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (batches shard "
@@ -316,6 +319,20 @@ def main() -> int:
     dev = torch.device("cuda:0")
     lib = kissfft_amd.load("float")
     stream = torch.cuda.Stream(device=dev)
+
+    if args.pmc_long_real:
+        n, b = 1 << 20, 256
+        with torch.cuda.stream(stream):
+            x = torch.rand(b, n, device=dev)
+            X = torch.empty(b, n // 2 + 1, 2, device=dev)
+            y = torch.empty_like(x)
+        fcfg, icfg = lib.fftr_alloc(n, False), lib.fftr_alloc(n, True)
+        for _ in range(3):
+            lib.fftr_batch(fcfg, x, X, b, n, n // 2 + 1, stream=stream)
+            lib.fftri_batch(icfg, X, y, b, n // 2 + 1, n, stream=stream)
+        stream.synchronize()
+        print(json.dumps({"pmc_long_real": True, "n": n, "signals": b}), flush=True)
+        return 0
 
     bufs = {}
     with torch.cuda.stream(stream):
