@@ -135,7 +135,9 @@ const std::vector<TileVariant> &tile_registry()
         add_rtile<16, 32, 8, 16>(*v, "rtile128_816_t16x32");
         add_rtile<16, 16, 16, 16>(*v, "rtile256_1616_t16x16");
         add_rtile<16, 32, 16, 16>(*v, "rtile256_1616_t16x32");
-        // real 2^20 x512: t64x8 1.70 / 1.84 TB/s (r2c / c2r), t64x16 1.60 / 1.64
+        // real 2^20 x512 (r2c / c2r TB/s): t32x16 1.78 / 1.93, t64x8 1.69 / 1.84, t64x16
+        // 1.60 / 1.65; t32x32, t16x16, t32x8 no better than t64x8
+        add_rtile<32, 16, 8, 8, 8>(*v, "rtile512_888_t32x16");
         add_rtile<64, 8, 8, 8, 8>(*v, "rtile512_888_t64x8");
         add_rtile<64, 16, 8, 8, 8>(*v, "rtile512_888_t64x16");
         add_rtile<64, 8, 16, 16, 4>(*v, "rtile1024_16164_t64x8");
