@@ -184,6 +184,33 @@ def other_configs(lib, stream, dev) -> dict:
         kinds_lengths="c2c 256/1000/1009/1200/120, c2c inverse 3072, r2c 4096, c2r 4000",
         launch=mb.info())
     mb.close()
+    del items, calls
+
+    # fast FIR (overlap-scrap, kiss_fastfir): 2^28 real samples (1 GiB) through a 1000-tap
+    # filter, one call on device buffers, with the reference's default block (the next power
+    # of two >= 2 (taps - 1): 2048, each sample read 2048 / 1049 times) and with 8192
+    nsamp, ntap = 1 << 28, 1000
+    h = np.random.default_rng(13).standard_normal(ntap).astype(np.float32)
+    with torch.cuda.stream(stream):
+        fin = torch.rand(nsamp, device=dev) * 2 - 1
+        fout = torch.empty(nsamp, device=dev)
+    torch.cuda.synchronize()
+    for nfft in (0, 8192):
+        ff = lib.fastfir_alloc(h, real=True, nfft=nfft)
+        written = [0]
+
+        def fir():
+            ff.offset = 0
+            written[0] = ff.filter(fin, fout, nsamp)
+        # the kiss_fastfir call is synchronous (results valid on return): wall time per call
+        fir()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            fir()
+        t = (time.perf_counter() - t0) / 5
+        rec(f"fastfir_real_2p28_samples_1000_taps_nfft{ff.nfft}", t, 4 * nsamp + 4 * written[0],
+            written[0] // ff.ngood, samples_per_s=round(nsamp / t, 1), good_per_block=ff.ngood)
+    del fin, fout
     return out
 
 

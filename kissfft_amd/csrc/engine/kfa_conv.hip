@@ -37,8 +37,10 @@ static int conv_plan(int dev, int nfft, int real, const void *H, const ConvPlan 
     }
     ConvPlan cp;
     for (const ConvVariant &v : conv_registry())
-        if (v.N == nfft && v.real == real)
+        if (v.N == nfft && v.real == real) {  // the first registered of a length
             cp.v = &v;
+            break;
+        }
     if (!cp.v)
         return fail(-2, "fast convolution: no kernel for block length %d (powers of two 256..8192)", nfft);
     const std::vector<unsigned char> tf = float_twiddles(nfft, 0, nfft, 1), ti = float_twiddles(nfft, 1, nfft, 1);

@@ -322,6 +322,7 @@ const std::vector<ConvVariant> &conv_registry()
         add_conv<64, 2, 16, 16, 4>(*v, "conv1024_16164_t64x2");
         add_conv<128, 2, 16, 16, 8>(*v, "conv2048_16168_t128x2");
         add_conv<256, 1, 16, 16, 16>(*v, "conv4096_161616_t256");
+        add_conv<512, 1, 16, 16, 8, 4>(*v, "conv8192_161684_t512");
         add_conv<256, 1, 16, 16, 8, 4>(*v, "conv8192_161684_t256");
         return v;
     }();
