@@ -9,8 +9,8 @@
 //     walks the same columns) and of the first pass of the four-step split;
 //   * "row" phases: consecutive lanes take consecutive elements of one line.
 // In between, the stages of the radix schedule RS run LDS -> registers -> LDS exactly as
-// in the fast line kernel (FastStage, T threads per line).  Lines sit N+1 values apart in
-// LDS so that lanes working on neighbouring lines fall in different banks.
+// in the fast line kernel (FastStage, T threads per line).  Lines sit tile_pitch() values
+// apart in LDS so that lanes working on neighbouring lines fall in different banks.
 //
 // TWID multiplies element e of line l by W^(lo*e), W = exp(-+2*pi*i/tw_n), on the way out:
 // the twiddle step between the two passes of the four-step split (Bailey, "FFTs in
@@ -69,6 +69,16 @@ __device__ __forceinline__ long long tile_of_block(long long b, long long ntiles
     return ((s / G) * 8 + x) * G + s % G;
 }
 
+// Line pitch in LDS.  The column phases put lanes ln = 0..TL-1 on neighbouring lines and
+// the rest of a wave's lanes on the next elements e0: lane (ln, e0) hits bank pair
+// (ln * pitch + e0) mod 32, all different when pitch = N + 32 / TL (mod 32) for TL < 32
+// (N + 1, the usual odd pitch, puts lanes (ln, e0) and (ln + 1, e0 - 1) on one bank).
+template <int N, int TL>
+constexpr int tile_pitch()
+{
+    return N + (TL < 32 ? 32 / TL : 1);
+}
+
 template <class A, bool INV, int T, int TL, bool LCOL, bool SCOL, bool TWID, class RS>
 __global__ void __launch_bounds__(T *TL) k_tile(TileArgs a)
 {
@@ -76,7 +86,7 @@ __global__ void __launch_bounds__(T *TL) k_tile(TileArgs a)
     typedef Ops<A> O;
     typedef Cx<A> C;
     constexpr int N = RS::N();
-    constexpr int NS = N + 1;
+    constexpr int NS = tile_pitch<N, TL>();
     constexpr int IT = N / T;  // elements per thread in the HBM phases
     static_assert(N % T == 0, "threads per line must divide the length");
     static_assert(!TWID || SCOL, "the four-step twiddle is applied on column stores");

@@ -79,7 +79,7 @@ __global__ void __launch_bounds__(T *TL) k_tile_rsplit(RealTileArgs a)
     typedef Ops<F32> O;
     typedef Cx<F32> C;
     constexpr int N1 = RS::N();
-    constexpr int NS = N1 + 1;
+    constexpr int NS = N1 + 1;  // row loads: the odd pitch
     constexpr int IT = N1 / T;
     constexpr int NP = TL / 2;                // row pairs per tile
     constexpr int IS = NP * N1 / (T * TL);    // split items per thread
@@ -192,7 +192,7 @@ __global__ void __launch_bounds__(T *TL) k_tile_rmerge(RealTileArgs a)
     typedef Ops<F32> O;
     typedef Cx<F32> C;
     constexpr int N2 = RS::N();
-    constexpr int NS = N2 + 1;
+    constexpr int NS = tile_pitch<N2, TL>();  // column loads: see k_tile
     constexpr int IT = N2 / T;
     constexpr int NP = TL / 2;              // column pairs per tile
     constexpr int IM = NP * N2 / (T * TL);  // merge items per thread

@@ -21,7 +21,7 @@ namespace kfa {
 template <bool INV, int T, int TL, bool LCOL, bool SCOL, bool TWID, class RS>
 static int launch_tile(const TileArgs &a, long long blocks, hipStream_t s)
 {
-    const size_t lds = (size_t)TL * (RS::N() + 1) * sizeof(float2);
+    const size_t lds = (size_t)TL * tile_pitch<RS::N(), TL>() * sizeof(float2);
     static std::once_flag once;
     std::call_once(once, [=] {
         (void)hipFuncSetAttribute((const void *)k_tile<F32, INV, T, TL, LCOL, SCOL, TWID, RS>,
@@ -40,7 +40,7 @@ static void add_tile(std::vector<TileVariant> &v, const char *name)
     constexpr int G = 2;  // 16-byte stage-table rows of float2
     typedef Radices<G, R...> RS;
     static_assert(T * TL <= 1024, "workgroup too large");
-    static_assert((size_t)TL * (RS::N() + 1) * 8 <= 160 * 1024, "tile exceeds LDS");
+    static_assert((size_t)TL * tile_pitch<RS::N(), TL>() * 8 <= 160 * 1024, "tile exceeds LDS");
     for (int mode = 0; mode < 3; ++mode)
         for (int inv = 0; inv < 2; ++inv) {
             TileVariant f;
@@ -52,7 +52,7 @@ static void add_tile(std::vector<TileVariant> &v, const char *name)
             f.radices = {R...};
             f.tw_pad = G;
             f.tw_len = RS::tw_len();
-            f.lds = (size_t)TL * (RS::N() + 1) * 8;
+            f.lds = (size_t)TL * tile_pitch<RS::N(), TL>() * 8;
             f.name = name;
             f.launch_real = nullptr;
             if (mode == TILE_COLS)
@@ -71,7 +71,7 @@ static void add_tile(std::vector<TileVariant> &v, const char *name)
 template <bool MERGE, int T, int TL, class RS>
 static int launch_rtile(const RealTileArgs &a, long long blocks, hipStream_t s)
 {
-    const size_t lds = (size_t)TL * (RS::N() + 1) * sizeof(float2);
+    const size_t lds = (size_t)TL * (MERGE ? tile_pitch<RS::N(), TL>() : RS::N() + 1) * sizeof(float2);
     const void *fn = MERGE ? (const void *)k_tile_rmerge<T, TL, RS> : (const void *)k_tile_rsplit<T, TL, RS>;
     static std::once_flag once;
     std::call_once(once, [=] { (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); });
@@ -92,7 +92,7 @@ static void add_rtile(std::vector<TileVariant> &v, const char *name)
     constexpr int G = 2;
     typedef Radices<G, R...> RS;
     static_assert(T * TL <= 1024, "workgroup too large");
-    static_assert((size_t)TL * (RS::N() + 1) * 8 + (size_t)TL * (RS::N() / T) * 8 + 8 <= 160 * 1024,
+    static_assert((size_t)TL * tile_pitch<RS::N(), TL>() * 8 + (size_t)TL * (RS::N() / T) * 8 + 8 <= 160 * 1024,
                   "tile exceeds LDS");
     for (int inv = 0; inv < 2; ++inv) {
         TileVariant f;
@@ -104,7 +104,7 @@ static void add_rtile(std::vector<TileVariant> &v, const char *name)
         f.radices = {R...};
         f.tw_pad = G;
         f.tw_len = RS::tw_len();
-        f.lds = (size_t)TL * (RS::N() + 1) * 8;
+        f.lds = (size_t)TL * (inv ? tile_pitch<RS::N(), TL>() : RS::N() + 1) * 8;
         f.name = name;
         f.launch = nullptr;
         f.launch_real = inv ? &launch_rtile<true, T, TL, RS> : &launch_rtile<false, T, TL, RS>;
