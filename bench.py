@@ -218,8 +218,9 @@ def serving(lib, dev, count: int, rate: float) -> dict:
     """The serving stream: `count` seeded requests (c2c / r2c / inverse, n in {256, 1000,
     1009, 3072, 65536, 1048576}, batch log-uniform in [1, 4096]) through one request server.
 
-    open loop:  Poisson arrivals at `rate` per second, submitted at their arrival times;
-                p50 / p99 latency (submit to completion) under that offered load.
+    open loop:  Poisson arrivals at `rate` per second (and the same arrivals at half that
+                rate), submitted at their arrival times; p50 / p99 latency (submit to
+                completion) under that offered load.
     saturated:  the same requests all submitted at once; throughput = algorithmic bytes /
                 (first submit to last completion).
     per-call:   the same requests as a loop of single-transform kiss_fft / kiss_fftr /
@@ -251,12 +252,13 @@ def serving(lib, dev, count: int, rate: float) -> dict:
         srv.wait(srv.submit(kinds[key[0]], key[1], ibufs[key], warm_out, 1))
     srv.reset_stats()
 
-    def run(open_loop: bool):
+    def run(open_loop: bool, stretch: float = 1.0):
+        # stretch > 1: the same arrivals spread over stretch times as long (rate / stretch)
         t0 = time.perf_counter()
         tickets = []
         for r, ob in zip(reqs, obufs):
             if open_loop:
-                while time.perf_counter() - t0 < r["t"]:
+                while time.perf_counter() - t0 < r["t"] * stretch:
                     pass
             tickets.append(srv.submit(kinds[r["kind"]], r["n"], ibufs[(r["kind"], r["n"])], ob, r["batch"]))
         for t in tickets:
@@ -266,6 +268,7 @@ def serving(lib, dev, count: int, rate: float) -> dict:
         return st
 
     ol = run(True)
+    ol_half = run(True, 2.0)
     sat = run(False)
     sample = [0, len(reqs) // 2, len(reqs) - 1]
     served = {i: obufs[i][: W.request_bytes(reqs[i]["kind"], reqs[i]["n"], 1)[1] // 4].cpu() for i in sample}
@@ -303,6 +306,10 @@ def serving(lib, dev, count: int, rate: float) -> dict:
         "open_loop": {"rate_per_s": rate, "offered_GBps": round(total_bytes / reqs[-1]["t"] / 1e9, 1),
                       "p50_ms": ms(ol["p50_s"]), "p99_ms": ms(ol["p99_s"]), "max_ms": ms(ol["max_s"]),
                       "GBps": round(total_bytes / ol["span_s"] / 1e9, 1), "launches": ol["launches"]},
+        "open_loop_half_rate": {"rate_per_s": rate / 2, "offered_GBps": round(total_bytes / reqs[-1]["t"] / 2e9, 1),
+                                "p50_ms": ms(ol_half["p50_s"]), "p99_ms": ms(ol_half["p99_s"]),
+                                "max_ms": ms(ol_half["max_s"]), "GBps": round(total_bytes / ol_half["span_s"] / 1e9, 1),
+                                "launches": ol_half["launches"]},
         "saturated": {"GBps": round(sat_gbps, 1), "transforms_per_s": round(transforms / sat["span_s"], 1),
                       "span_ms": ms(sat["span_s"]), "p50_ms": ms(sat["p50_s"]), "p99_ms": ms(sat["p99_s"]),
                       "launches": sat["launches"]},
