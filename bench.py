@@ -115,6 +115,17 @@ def other_configs(lib, stream, dev) -> dict:
     rec("c2c_q15_512_x1048576", t, B * 8 * 512, B)
     del xq, yq
 
+    # Q31 complex N=512, 2^19 transforms (the same 2 GiB per side as the Q15 line)
+    l31 = kissfft_amd.load("int32_t")
+    B = 1 << 19
+    with torch.cuda.stream(stream):
+        x31 = torch.randint(-(1 << 30), 1 << 30, (B, 512, 2), dtype=torch.int32, device=dev)
+        y31 = torch.empty_like(x31)
+    c31 = l31.fft_alloc(512)
+    t = _time(lambda: l31.fft_batch(c31, x31, y31, B, 512, 512, stream=stream), stream, 5)
+    rec("c2c_q31_512_x524288", t, B * 16 * 512, B)
+    del x31, y31
+
     # 3-D complex float 64x96x120, 16 transforms
     dims = (64, 96, 120)
     with torch.cuda.stream(stream):

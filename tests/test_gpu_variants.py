@@ -3,7 +3,7 @@
 The engine times the variants of a size on first use and keeps the fastest; this test
 does not rely on that choice: it forces each variant by name (KISSFFT_AMD_VARIANT),
 drops the plan cache between variants (kiss_fft_cleanup), and checks the result --
-bit-exact for Q15, rel-L2 for float.  The variant list is read from the registry source.
+bit-exact for Q15 and Q31, rel-L2 for float.  The variant list is read from the registry source.
 """
 import os
 import re
@@ -55,32 +55,35 @@ def test_registry_parsed():
 
 
 @pytest.mark.parametrize("typ,kind,n,name", CASES, ids=[f"{c[3]}-{c[1]}" for c in CASES])
-def test_forced_variant(typ, kind, n, name, monkeypatch):
-    datatype = {"F32": "float", "Q15": "int16_t", "Q15P": "int16_t"}[typ]
+def test_forced_variant(typ, kind, n, name, monkeypatch, capfd):
+    datatype = {"F32": "float", "Q15": "int16_t", "Q15P": "int16_t", "Q31": "int32_t"}[typ]
     lib = kissfft_amd.load(datatype)
     lib.L.kiss_fft_cleanup()
     monkeypatch.setenv("KISSFFT_AMD_VARIANT", name)
+    monkeypatch.setenv("KISSFFT_AMD_VERBOSE", "1")
     say("variant", name, kind, n)
     rng = np.random.default_rng(hash(name) & 0xFFFF)
     B = 37  # not a multiple of any lines-per-workgroup: exercises the partial last workgroup
     try:
-        if datatype == "int16_t":
-            O = Oracle("int16_t")
+        if datatype in ("int16_t", "int32_t"):
+            O = Oracle(datatype)
+            dt = np.int16 if datatype == "int16_t" else np.int32
+            lo, hi = (-32768, 32768) if datatype == "int16_t" else (-(1 << 31), 1 << 31)
             if kind == "c2c":
-                x = rng.integers(-32768, 32768, (B, n, 2)).astype(np.int16)
+                x = rng.integers(lo, hi, (B, n, 2)).astype(dt)
                 for inv in (False, True):
                     cfg = lib.fft_alloc(n, inv)
                     y = np.zeros_like(x)
                     lib.fft_batch(cfg, x, y, B, n, n)
                     np.testing.assert_array_equal(y, O.fft(x, inv))
             elif kind == "r2c":
-                x = rng.integers(-32768, 32768, (B, 2 * n)).astype(np.int16)
-                y = np.zeros((B, n + 1, 2), np.int16)
+                x = rng.integers(lo, hi, (B, 2 * n)).astype(dt)
+                y = np.zeros((B, n + 1, 2), dt)
                 lib.fftr_batch(lib.fftr_alloc(2 * n), x, y, B, 2 * n, n + 1)
                 np.testing.assert_array_equal(y, O.fftr(x))
             else:
-                X = rng.integers(-32768, 32768, (B, n + 1, 2)).astype(np.int16)
-                y = np.zeros((B, 2 * n), np.int16)
+                X = rng.integers(lo, hi, (B, n + 1, 2)).astype(dt)
+                y = np.zeros((B, 2 * n), dt)
                 lib.fftri_batch(lib.fftr_alloc(2 * n, True), X, y, B, n + 1, 2 * n)
                 np.testing.assert_array_equal(y, O.fftri(X, 2 * n))
         else:
@@ -104,6 +107,8 @@ def test_forced_variant(typ, kind, n, name, monkeypatch):
                 lib.fftri_batch(lib.fftr_alloc(2 * n, True), X, y, B, n + 1, 2 * n)
                 want = np.fft.irfft(Xc.astype(np.complex128), n=2 * n, axis=-1) * 2 * n
                 assert rel_l2(y, want) <= 4e-7
+        # the forced variant is the one that ran (a name the plan cannot take is ignored)
+        assert f"fast variant {name}\n" in capfd.readouterr().err, name
     finally:
         lib.L.kiss_fft_cleanup()
 
