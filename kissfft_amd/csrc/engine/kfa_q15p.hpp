@@ -114,12 +114,25 @@ __device__ __forceinline__ u32 mul_pi(u32 z) { return mul_lo(swap(z), 0x0001ffff
 
 __device__ __forceinline__ int lo16(u32 w) { return (int)(short)(w & 0xffffu); }
 
-// C_FIXDIV(c, p): K = floor(32767 / p)
+// a.lo*b.lo + a.hi*b.hi + 2^15 (the dot product above with twice its rounding constant)
+__device__ __forceinline__ int dot_h(u32 a, u32 b)
+{
+    int r;
+    asm("v_dot2_i32_i16 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(1 << 15));
+    return r;
+}
+
+// C_FIXDIV(c, p): K = floor(32767 / p), each component sround(c * K) truncated to int16.
+// 2K < 32768 for every p >= 2, so the products are taken with 2K and 2^15: the sum is
+// 2 * (c*K + 2^14) (no overflow: |c * 2K| < 2^30), whose high 16 bits are exactly the
+// rounded, truncated component; one byte permute packs the two high halves.
 template <int P>
 __device__ __forceinline__ u32 fixdiv(u32 c)
 {
-    constexpr int K = 32767 / P;
-    return scale(c, (u32)K, (u32)K << 16);
+    constexpr u32 K2 = 2 * (32767 / P);
+    static_assert(K2 < 32768, "2K must fit a signed 16-bit lane");
+    const int re = dot_h(c, K2), im = dot_h(c, K2 << 16);
+    return __builtin_amdgcn_perm((u32)im, (u32)re, 0x07060302u);  // (re.hi, im.hi)
 }
 
 // kiss_fft.c:15-36
