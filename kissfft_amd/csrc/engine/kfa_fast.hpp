@@ -610,9 +610,9 @@ struct LastBuf {
     static constexpr bool is_a = !DB || (K % 2 == 0);
 };
 
-// Dynamic LDS: (DB ? 2 : 1) * LPB * N complex values.
-template <class A, int KIND, bool INV, int T, int LPB, bool DB, bool FUSE, class RS>
-__global__ void __launch_bounds__(T *LPB) k_fast(FastArgs a)
+// The kernel body; UNIT: both element strides are 1.
+template <class A, int KIND, bool INV, int T, int LPB, bool DB, bool FUSE, class RS, bool UNIT>
+__device__ __forceinline__ void k_fast_body(const FastArgs &a)
 {
     typedef typename A::store2 S;
     typedef Ops<A> O;
@@ -631,6 +631,7 @@ __global__ void __launch_bounds__(T *LPB) k_fast(FastArgs a)
     line_base<S>(a, valid ? gl : 0, gin, gout);
     const S *stw = reinterpret_cast<const S *>(__builtin_assume_aligned(a.stw, 16));
     const S *rtw = reinterpret_cast<const S *>(a.rtw);
+    const long long ie = UNIT ? 1 : a.in_el, oe = UNIT ? 1 : a.out_el;  // element strides
 
     if constexpr (KIND == FK_C2R && !FUSE) {
         // Hermitian merge into bufA (kiss_fftr.c:131-153); bins read from HBM.  Every load
@@ -643,8 +644,8 @@ __global__ void __launch_bounds__(T *LPB) k_fast(FastArgs a)
         for (int j = 0; j < JK; ++j) {
             const int k = t + j * T;
             const int kc = k < NK ? k : NK - 1;
-            xa[j] = valid ? O::load(hbm_ld(gin + (long long)kc * a.in_el)) : O::zero();
-            bz[j] = valid ? O::load(hbm_ld(gin + (long long)(N - kc) * a.in_el)) : O::zero();
+            xa[j] = valid ? O::load(hbm_ld(gin + (long long)kc * ie)) : O::zero();
+            bz[j] = valid ? O::load(hbm_ld(gin + (long long)(N - kc) * ie)) : O::zero();
             tw[j] = O::load(rtw[kc > 0 ? kc - 1 : 0]);
         }
 #pragma unroll
@@ -671,8 +672,8 @@ __global__ void __launch_bounds__(T *LPB) k_fast(FastArgs a)
         __syncthreads();
     }
 
-    fast_run_all<A, INV, T, KIND, DB, FUSE, RS>(std::make_integer_sequence<int, K>{}, bufA, bufB, gin, a.in_el,
-                                                gout, a.out_el, stw, rtw, t, valid);
+    fast_run_all<A, INV, T, KIND, DB, FUSE, RS>(std::make_integer_sequence<int, K>{}, bufA, bufB, gin, ie,
+                                                gout, oe, stw, rtw, t, valid);
 
     if constexpr (KIND == FK_R2C && !FUSE) {
         // split (kiss_fftr.c:88-116) from the LDS buffer the last stage wrote
@@ -699,7 +700,7 @@ __global__ void __launch_bounds__(T *LPB) k_fast(FastArgs a)
             if (k == 0) {
                 C dc = O::fixdiv(za[j], 2);
                 hbm_st(gout, O::pack(C{O::add(dc.r, dc.i), O::zero().r}));
-                hbm_st(gout + (long long)N * a.out_el, O::pack(C{O::sub(dc.r, dc.i), O::zero().r}));
+                hbm_st(gout + (long long)N * oe, O::pack(C{O::sub(dc.r, dc.i), O::zero().r}));
                 continue;
             }
             C xa = O::fixdiv(za[j], 2);
@@ -707,10 +708,30 @@ __global__ void __launch_bounds__(T *LPB) k_fast(FastArgs a)
             C f1 = O::cadd(xa, xb), f2 = O::csub(xa, xb);
             C w = O::cmul(f2, tw[j]);
             if (2 * k != N)
-                hbm_st(gout + (long long)k * a.out_el, O::pack(C{O::half_add(f1.r, w.r), O::half_add(f1.i, w.i)}));
-            hbm_st(gout + (long long)(N - k) * a.out_el, O::pack(C{O::half_sub(f1.r, w.r), O::half_sub(w.i, f1.i)}));
+                hbm_st(gout + (long long)k * oe, O::pack(C{O::half_add(f1.r, w.r), O::half_add(f1.i, w.i)}));
+            hbm_st(gout + (long long)(N - k) * oe, O::pack(C{O::half_sub(f1.r, w.r), O::half_sub(w.i, f1.i)}));
         }
     }
+}
+
+// Dynamic LDS: (DB ? 2 : 1) * LPB * N complex values.
+// Fixed point: contiguous lines (both element strides 1, every call but kiss_fft_stride
+// and strided batches) take an instance whose element offsets are compile-time multiples
+// of the element size, folded into the loads' and stores' immediate offsets; any other
+// stride multiplies 64-bit indices per element.  The fixed-point kernels are bound by
+// vector-ALU issue (Q15 c2c 512: +9-11 % from the unit instance, Q31 +4 %); the float kernels
+// stream from HBM and ran 1-3 % slower with the second instance (more registers), so they
+// keep the one general body.
+template <class A, int KIND, bool INV, int T, int LPB, bool DB, bool FUSE, class RS>
+__global__ void __launch_bounds__(T *LPB) k_fast(FastArgs a)
+{
+    if constexpr (A::fixed) {
+        if (a.in_el == 1 && a.out_el == 1) {
+            k_fast_body<A, KIND, INV, T, LPB, DB, FUSE, RS, true>(a);
+            return;
+        }
+    }
+    k_fast_body<A, KIND, INV, T, LPB, DB, FUSE, RS, false>(a);
 }
 
 }  // namespace kfa
