@@ -348,6 +348,9 @@ def main() -> int:
     ap.add_argument("--pmc-long-real", action="store_true",
                     help="for rocprofv3: 256 real signals of 2^20 points, forward and inverse three "
                          "times each (the fused four-step real passes), and exit")
+    ap.add_argument("--pmc-q15", action="store_true",
+                    help="for rocprofv3: Q15 c2c N=512 over 2^20 transforms, three launches after "
+                         "the plan's first call, and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (batches shard "
@@ -377,6 +380,20 @@ def main() -> int:
             lib.fftri_batch(icfg, X, y, b, n // 2 + 1, n, stream=stream)
         stream.synchronize()
         print(json.dumps({"pmc_long_real": True, "n": n, "signals": b}), flush=True)
+        return 0
+
+    if args.pmc_q15:
+        lq = kissfft_amd.load("int16_t")
+        b = 1 << 20
+        with torch.cuda.stream(stream):
+            xq = torch.randint(-16384, 16384, (b, 512, 2), dtype=torch.int16, device=dev)
+            yq = torch.empty_like(xq)
+        cq = lq.fft_alloc(512)
+        lq.fft_batch(cq, xq, yq, b, 512, 512, stream=stream)  # plan + variant choice
+        for _ in range(3):
+            lq.fft_batch(cq, xq, yq, b, 512, 512, stream=stream)
+        stream.synchronize()
+        print(json.dumps({"pmc_q15": True, "n": 512, "transforms": b, "alg_bytes": b * 8 * 512}), flush=True)
         return 0
 
     bufs = {}
