@@ -70,6 +70,46 @@ def _run(cmd: list[str], verbose: bool) -> None:
             sys.stderr.write("\n".join(lines) + "\n")
 
 
+# Every kernel must keep its data in registers and LDS: a kernel that spills to scratch
+# (private memory in HBM) runs several times slower, and a change that causes one -- an
+# argument block copied to the stack, say -- is easy to miss.  The engine is compiled with
+# the kernel resource-usage remarks and the build fails on any kernel with scratch.
+# The exceptions are the general line kernels (k_lines, and k_multi, its form for the
+# multi-batch): their radix schedule is chosen at run time (any radix, the generic DFT
+# included), so the butterfly operands sit in a local array indexed at run time (96 bytes
+# per lane); they serve the lengths and layouts no compiled kernel covers.
+RESOURCE_REMARKS = ["-Xarch_device", "-Rpass-analysis=kernel-resource-usage"]
+SCRATCH_ALLOWED = ("_ZN3kfa7k_lines", "_ZN3kfa7k_multi")
+
+
+def _scratch_kernels(remarks: str) -> list[tuple[str, int]]:
+    """(kernel, bytes per lane) for every kernel the remarks report with scratch."""
+    out, name = [], None
+    for line in remarks.splitlines():
+        if "remark: Function Name: " in line:
+            name = line.split("remark: Function Name: ", 1)[1].split(" [-Rpass")[0].strip()
+        elif "remark:     ScratchSize [bytes/lane]: " in line and name:
+            n = int(line.split("ScratchSize [bytes/lane]: ", 1)[1].split()[0])
+            if n and not name.startswith(SCRATCH_ALLOWED):
+                out.append((name, n))
+    return out
+
+
+def _run_engine(cmd: list[str], verbose: bool) -> int:
+    """Compiles one engine source; returns the number of kernels it reports."""
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run([*cmd, *RESOURCE_REMARKS], capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"build step failed ({r.returncode}): {' '.join(cmd[:3])} ...")
+    bad = _scratch_kernels(r.stderr)
+    if bad:
+        lines = "\n".join(f"  {n} bytes/lane  {k}" for k, n in bad)
+        raise RuntimeError(f"kernels use scratch memory (registers spilled or locals on the stack):\n{lines}")
+    return r.stderr.count("remark: Function Name: ")
+
+
 def lib_path(datatype: str) -> str:
     return os.path.join(LIBDIR, f"libkissfft-{datatype}.so")
 
@@ -84,8 +124,9 @@ def build(datatypes=None, verbose: bool = True) -> list[str]:
     # the HIP translation units are independent: compile them side by side
     workers = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "0") or 0) or (os.cpu_count() or 1)))
     with concurrent.futures.ThreadPoolExecutor(max_workers=workers) as pool:
-        for f in [pool.submit(_run, c, verbose) for c in cmds]:
-            f.result()
+        nkernels = sum(f.result() for f in [pool.submit(_run_engine, c, verbose) for c in cmds])
+    if verbose:
+        print(f"{nkernels} kernels; none but the general line kernels use scratch memory", flush=True)
     built = []
     for dt in datatypes:
         host_objs = []

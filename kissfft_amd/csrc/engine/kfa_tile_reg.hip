@@ -140,8 +140,8 @@ const std::vector<TileVariant> &tile_registry()
         add_rtile<32, 16, 8, 8, 8>(*v, "rtile512_888_t32x16");
         add_rtile<64, 8, 8, 8, 8>(*v, "rtile512_888_t64x8");
         add_rtile<64, 16, 8, 8, 8>(*v, "rtile512_888_t64x16");
+        // (1024 threads x 16 lines of 1024 points spills 40 bytes per lane: not registered)
         add_rtile<64, 8, 16, 16, 4>(*v, "rtile1024_16164_t64x8");
-        add_rtile<64, 16, 16, 16, 4>(*v, "rtile1024_16164_t64x16");
         return v;
     }();
     return *reg;

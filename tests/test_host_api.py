@@ -198,3 +198,22 @@ def test_cpp_class_header_compiles():
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
                         os.path.join(ROOT, "tests", "cpp", "facade_driver.cpp")], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_build_rejects_kernels_with_scratch():
+    """The build reads the compiler's kernel resource-usage remarks and fails on a kernel
+    with scratch memory (a spill or a stack copy), except the general line kernels, whose
+    run-time radix needs a run-time-indexed local array."""
+    from kissfft_amd import build as B
+
+    tail = " [-Rpass-analysis=kernel-resource-usage]"
+    def remarks(name, scratch):
+        return (f"x.hpp:1:1: remark: Function Name: {name}{tail}\n"
+                f"x.hpp:1:1: remark:     VGPRs: 40{tail}\n"
+                f"x.hpp:1:1: remark:     ScratchSize [bytes/lane]: {scratch}{tail}\n")
+    text = (remarks("_ZN3kfa6k_fastINS_4Q15PEEEvNS_8FastArgsE", 1624)
+            + remarks("_ZN3kfa6k_fastINS_3F32EEEvNS_8FastArgsE", 0)
+            + remarks("_ZN3kfa7k_linesINS_3F32EEEvPKNT_6store2E", 96)
+            + remarks("_ZN3kfa7k_multiINS_3F32EEEvPKNS_9MultiItemEix", 96))
+    assert B._scratch_kernels(text) == [("_ZN3kfa6k_fastINS_4Q15PEEEvNS_8FastArgsE", 1624)]
+    assert B._scratch_kernels(remarks("_ZN3kfa6k_tileE", 0)) == []
