@@ -117,9 +117,13 @@ __device__ __forceinline__ int lo16(u32 w) { return (int)(short)(w & 0xffffu); }
 // a.lo*b.lo + a.hi*b.hi + 2^15 (the dot product above with twice its rounding constant)
 __device__ __forceinline__ int dot_h(u32 a, u32 b)
 {
+#if KFA_Q15_DOT_ASM
     int r;
     asm("v_dot2_i32_i16 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(1 << 15));
     return r;
+#else
+    return __builtin_amdgcn_sdot2(as_v(a), as_v(b), 1 << 15, false);
+#endif
 }
 
 // C_FIXDIV(c, p): K = floor(32767 / p), each component sround(c * K) truncated to int16.
