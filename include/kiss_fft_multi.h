@@ -18,8 +18,10 @@
  *   - items longer than one LDS line (float 8192 complex / 16384 real, Q15 16384 / 32768)
  *     run as their own multi-pass launches; a multi-batch holding such items cannot be
  *     captured (they allocate scratch).
- * The launches are spread over the multi-batch's own streams and joined on the caller's
- * stream, so independent groups overlap; capture records the same fork / join.
+ * The launches run back to back on the caller's stream and capture records them into one
+ * graph (KISSFFT_AMD_MULTI_HELPERS=n at creation forks them over n streams of the
+ * multi-batch, joined on the caller's stream; the hand-overs between queues cost more than
+ * the launches overlap).
  * KISSFFT_AMD_MULTI_PERSISTENT=1 at creation puts every single-pass item into the
  * persistent launch.  Results equal the per-item batched calls of the fixed-point builds
  * bit for bit, and the float builds' within the float tolerance.  Items must not share
