@@ -593,6 +593,61 @@ __device__ __forceinline__ void fast_run_one(typename A::store2 *bufA, typename 
         __syncthreads();
 }
 
+// Stages 0 and 1 without the LDS exchange between them.  Thread t owns the stage-0
+// butterflies b = t + j*T, whose outputs sit at b + q*NB0; its stage-1 butterflies
+// b' = t + j'*T read u*P1*p1 + r*P1 + o with u = b' / P1, o = b' % P1.  When T divides NB0
+// and P1, u = j'*T / P1 and o = (j'*T) % P1 + t, so every stage-1 input of the thread is
+// the output (j, q) = ((c % NB0) / T, c / NB0), c = u*P1*p1 + r*P1 + (j'*T) % P1, of one
+// of its own stage-0 butterflies: a permutation of registers fixed at compile time.  The
+// arithmetic of both stages is unchanged; one LDS store, one LDS gather and a barrier go.
+// Used by the packed Q15 complex kernels (bound by instruction issue), whose 512-point
+// reference schedule (2, 4, 4, 4, 4) meets the condition at 16, 32 and 64 threads per line.
+template <class A, bool INV, int T, int KIND, bool FUSE, class RS>
+struct FirstPairInRegisters {
+    typedef FastStage<A, INV, T, RS, 0, false> F0;
+    typedef FastStage<A, INV, T, RS, (RS::K > 1 ? 1 : 0), false> F1;
+    static constexpr bool value = IsPacked<A>::value && KIND == FK_C2C && !FUSE && RS::K > 2 &&
+                                  F0::NB % T == 0 && F1::P % T == 0 && F1::NB % T == 0;
+};
+
+template <class A, bool INV, int T, int KIND, bool DB, class RS>
+__device__ __forceinline__ void fast_run_first_pair(typename A::store2 *bufA, typename A::store2 *bufB,
+                                                    const typename A::store2 *gin, long long in_el,
+                                                    const typename A::store2 *stw, int t, bool valid)
+{
+    typedef FastStage<A, INV, T, RS, 0, false> F0;
+    typedef FastStage<A, INV, T, RS, 1, false> F1;
+    typename F0::C w[F0::J][F0::p];
+    F0::template gather<true>(w, gin, in_el, t, valid);
+    F0::compute(w, stw, t);
+    typename F1::C v[F1::J][F1::p];
+#pragma unroll
+    for (int j1 = 0; j1 < F1::J; ++j1) {
+#pragma unroll
+        for (int r = 0; r < F1::p; ++r) {
+            const int c = (j1 * T / F1::P) * F1::P * F1::p + r * F1::P + (j1 * T) % F1::P;
+            v[j1][r] = w[(c % F0::NB) / T][c / F0::NB];
+        }
+    }
+    F1::compute(v, stw, t);
+    // stage 1 writes where it always does: bufA, single-buffered or double-buffered (odd
+    // stages write bufA), which stage 2 reads; stage 1 is not the last stage (RS::K > 2)
+    F1::template store<false>(v, bufA, 1, t, valid);
+    (void)bufB;
+    __syncthreads();
+}
+
+template <class A, bool INV, int T, int KIND, bool DB, bool FUSE, class RS, int S, int FROM>
+__device__ __forceinline__ void fast_run_from(typename A::store2 *bufA, typename A::store2 *bufB,
+                                              const typename A::store2 *gin, long long in_el,
+                                              typename A::store2 *gout, long long out_el,
+                                              const typename A::store2 *stw, const typename A::store2 *rtw, int t,
+                                              bool valid)
+{
+    if constexpr (S >= FROM)
+        fast_run_one<A, INV, T, KIND, DB, FUSE, RS, S>(bufA, bufB, gin, in_el, gout, out_el, stw, rtw, t, valid);
+}
+
 template <class A, bool INV, int T, int KIND, bool DB, bool FUSE, class RS, int... S>
 __device__ __forceinline__ void fast_run_all(std::integer_sequence<int, S...>, typename A::store2 *bufA,
                                              typename A::store2 *bufB, const typename A::store2 *gin,
@@ -600,8 +655,14 @@ __device__ __forceinline__ void fast_run_all(std::integer_sequence<int, S...>, t
                                              const typename A::store2 *stw, const typename A::store2 *rtw, int t,
                                              bool valid)
 {
-    (fast_run_one<A, INV, T, KIND, DB, FUSE, RS, S>(bufA, bufB, gin, in_el, gout, out_el, stw, rtw, t, valid),
-     ...);
+    if constexpr (FirstPairInRegisters<A, INV, T, KIND, FUSE, RS>::value) {
+        fast_run_first_pair<A, INV, T, KIND, DB, RS>(bufA, bufB, gin, in_el, stw, t, valid);
+        (fast_run_from<A, INV, T, KIND, DB, FUSE, RS, S, 2>(bufA, bufB, gin, in_el, gout, out_el, stw, rtw, t, valid),
+         ...);
+    } else {
+        (fast_run_one<A, INV, T, KIND, DB, FUSE, RS, S>(bufA, bufB, gin, in_el, gout, out_el, stw, rtw, t, valid),
+         ...);
+    }
 }
 
 // buffer holding the result of the last stage (input of the real split)
