@@ -600,13 +600,13 @@ __device__ __forceinline__ void fast_run_one(typename A::store2 *bufA, typename 
 // the output (j, q) = ((c % NB0) / T, c / NB0), c = u*P1*p1 + r*P1 + (j'*T) % P1, of one
 // of its own stage-0 butterflies: a permutation of registers fixed at compile time.  The
 // arithmetic of both stages is unchanged; one LDS store, one LDS gather and a barrier go.
-// Used by the packed Q15 complex kernels (bound by instruction issue), whose 512-point
+// Used by the fixed-point complex kernels (bound by instruction issue), whose 512-point
 // reference schedule (2, 4, 4, 4, 4) meets the condition at 16, 32 and 64 threads per line.
 template <class A, bool INV, int T, int KIND, bool FUSE, class RS>
 struct FirstPairInRegisters {
     typedef FastStage<A, INV, T, RS, 0, false> F0;
     typedef FastStage<A, INV, T, RS, (RS::K > 1 ? 1 : 0), false> F1;
-    static constexpr bool value = IsPacked<A>::value && KIND == FK_C2C && !FUSE && RS::K > 2 &&
+    static constexpr bool value = A::fixed && KIND == FK_C2C && !FUSE && RS::K > 2 &&
                                   F0::NB % T == 0 && F1::P % T == 0 && F1::NB % T == 0;
 };
 
