@@ -169,3 +169,41 @@ def test_many_small_batches_grouped():
         got = got if k == C2R else cplx(got)
         assert rel_l2(got, fp64_ref(k, n, x.cpu().numpy())) <= ABS_FLOAT_BOUND, (k, n, b)
     m.close()
+
+
+def test_helper_streams_same_results(monkeypatch):
+    """KISSFFT_AMD_MULTI_HELPERS=n forks the launches over n streams of the multi-batch and
+    joins them on the caller's stream: eager execution and graph replays give the same bits
+    as the default single-stream form."""
+    import torch
+    lib = kissfft_amd.load("float")
+    dev = torch.device("cuda:0")
+    items = [(C2C, 256, 40), (C2C, 1009, 3), (R2C, 4096, 6), (C2R, 2000, 5), (C2C_INV, 3072, 4), (C2C, 60, 90)]
+    bufs = [make(torch, dev, k, n, b, 500 + i, "float") for i, (k, n, b) in enumerate(items)]
+    spec = [(k, n, x, y, b) for (k, n, b), (x, y) in zip(items, bufs)]
+    monkeypatch.setenv("KISSFFT_AMD_MULTI_HELPERS", "0")
+    single = lib.multi(spec)
+    single.exec()
+    want = [y.clone() for _, y in bufs]
+    single.close()
+    monkeypatch.setenv("KISSFFT_AMD_MULTI_HELPERS", "2")
+    forked = lib.multi(spec)
+    for _, y in bufs:
+        y.fill_(-3.0)
+    torch.cuda.synchronize()
+    forked.exec()
+    for (k, n, b), (_, y), w in zip(items, bufs, want):
+        say("helpers eager", k, n)
+        assert torch.equal(y, w), (k, n)
+    st = torch.cuda.Stream()
+    forked.capture(st)
+    for rep in range(2):
+        with torch.cuda.stream(st):
+            for _, y in bufs:
+                y.fill_(7.0)
+        forked.replay(st)
+        st.synchronize()
+        for (k, n, b), (_, y), w in zip(items, bufs, want):
+            say("helpers replay", rep, k, n)
+            assert torch.equal(y, w), (rep, k, n)
+    forked.close()
