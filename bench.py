@@ -225,7 +225,7 @@ def other_configs(lib, stream, dev) -> dict:
     return out
 
 
-def serving(lib, dev, count: int, rate: float) -> dict:
+def serving(lib, dev, count: int, rate: float, streams: int = 0) -> dict:
     """The serving stream: `count` seeded requests (c2c / r2c / inverse, n in {256, 1000,
     1009, 3072, 65536, 1048576}, batch log-uniform in [1, 4096]) through one request server.
 
@@ -256,7 +256,7 @@ def serving(lib, dev, count: int, rate: float) -> dict:
     total_bytes = sum(sum(W.request_bytes(r["kind"], r["n"], r["batch"])) for r in reqs)
     transforms = sum(r["batch"] for r in reqs)
 
-    srv = lib.server()
+    srv = lib.server(streams=streams)
     # warm-up: plans, twiddles and variant choice of every (kind, n), one transform each
     for key in inputs:
         warm_out = torch.empty(W.request_bytes(key[0], key[1], 1)[1] // 4, device=dev)
@@ -342,6 +342,8 @@ def main() -> int:
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-stream measurement")
     ap.add_argument("--serve-requests", type=int, default=192)
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="open-loop arrivals per second")
+    ap.add_argument("--serve-streams", type=int, default=0,
+                    help="streams of the request server (0: the library default)")
     ap.add_argument("--pmc-run", action="store_true",
                     help="for rocprofv3 --pmc: one plain copy of a known size (the counter scale), "
                          "then one forward and one inverse N=4096 launch, and exit")
@@ -473,7 +475,7 @@ def main() -> int:
     if not args.no_serving:
         del bufs, x, X, y
         torch.cuda.empty_cache()
-        serve = serving(lib, dev, args.serve_requests, args.serve_rate)
+        serve = serving(lib, dev, args.serve_requests, args.serve_rate, args.serve_streams)
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic_r2c_4096.json")
