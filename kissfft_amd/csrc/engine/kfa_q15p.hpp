@@ -139,11 +139,58 @@ __device__ __forceinline__ u32 fixdiv(u32 c)
     return __builtin_amdgcn_perm((u32)im, (u32)re, 0x07060302u);  // (re.hi, im.hi)
 }
 
+// C_FIXDIV of K values and complex products of K values, each written as all of its dot
+// products first, then all of the packing: a packing instruction right behind the dot
+// product it reads waits a cycle (an s_nop the compiler inserts), one K - 1 products later
+// does not.  Same operations, same bits as fixdiv / cmul one at a time.
+template <int P, int K>
+__device__ __forceinline__ void fixdiv_n(u32 (&x)[K], const u32 *c)
+{
+    constexpr u32 K2 = 2 * (32767 / P);
+    int re[K], im[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        re[k] = dot_h(c[k], K2);
+        im[k] = dot_h(c[k], K2 << 16);
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        x[k] = __builtin_amdgcn_perm((u32)im[k], (u32)re[k], 0x07060302u);
+}
+
+template <int K>
+__device__ __forceinline__ void cmul_n(u32 (&s)[K], const u32 *a, const uint2 *tw)
+{
+    int re[K], im[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        re[k] = dot(a[k], tw[k].x);
+        im[k] = dot(a[k], tw[k].y);
+    }
+#if KFA_Q15_SDWA_PACK
+    // round_pack's two shifts, each kind for all K values in a row
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        asm("v_lshrrev_b32 %0, 15, %1" : "=v"(s[k]) : "v"(re[k]));
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        asm("v_lshrrev_b32_sdwa %0, 15, %1 dst_sel:WORD_1 dst_unused:UNUSED_PRESERVE src0_sel:DWORD src1_sel:DWORD"
+            : "+v"(s[k])
+            : "v"(im[k]));
+#else
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        s[k] = round_pack(re[k], im[k]);
+#endif
+}
+
 // kiss_fft.c:15-36
 __device__ __forceinline__ void bfly2(u32 *v, const uint2 *tw)
 {
-    u32 a = fixdiv<2>(v[0]), b = fixdiv<2>(v[1]);
-    u32 t = cmul(b, tw[0].x, tw[0].y);
+    u32 x[2];
+    fixdiv_n<2, 2>(x, v);
+    const u32 a = x[0];
+    u32 t = cmul(x[1], tw[0].x, tw[0].y);
     v[1] = sub(a, t);
     v[0] = add(a, t);
 }
@@ -152,10 +199,11 @@ __device__ __forceinline__ void bfly2(u32 *v, const uint2 *tw)
 template <bool INV>
 __device__ __forceinline__ void bfly4(u32 *v, const uint2 *tw)
 {
-    u32 x0 = fixdiv<4>(v[0]), x1 = fixdiv<4>(v[1]), x2 = fixdiv<4>(v[2]), x3 = fixdiv<4>(v[3]);
-    u32 s0 = cmul(x1, tw[0].x, tw[0].y);
-    u32 s1 = cmul(x2, tw[1].x, tw[1].y);
-    u32 s2 = cmul(x3, tw[2].x, tw[2].y);
+    u32 x[4], s[3];
+    fixdiv_n<4, 4>(x, v);
+    cmul_n<3>(s, x + 1, tw);
+    u32 x0 = x[0];
+    u32 s0 = s[0], s1 = s[1], s2 = s[2];
     u32 s5 = sub(x0, s1);
     x0 = add(x0, s1);
     u32 s3 = add(s0, s2);
