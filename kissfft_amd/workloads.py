@@ -63,6 +63,25 @@ def two_tone(seed: int, n: int, howmany: int, first: int = 0) -> np.ndarray:
     return (noise + tones[None, :]).astype(np.float32)
 
 
+def generate(fn, seed: int, n: int, howmany: int, workers: int = 8, chunk: int = 8192) -> np.ndarray:
+    """fn(seed, n, howmany) for large batches: slices of `chunk` items generated side by side
+    (the stream is position-addressable, so every slice is regenerated on its own and the
+    result equals one call of fn)."""
+    import concurrent.futures
+
+    first = fn(seed, n, min(chunk, howmany), 0)
+    out = np.empty((howmany,) + first.shape[1:], first.dtype)
+    out[:first.shape[0]] = first
+
+    def one(b0: int) -> None:
+        m = min(chunk, howmany - b0)
+        out[b0:b0 + m] = fn(seed, n, m, b0)
+
+    with concurrent.futures.ThreadPoolExecutor(max_workers=max(1, workers)) as pool:
+        list(pool.map(one, range(first.shape[0], howmany, chunk)))
+    return out
+
+
 # The reference's benchmark workloads, one GPU each.
 CONFIGS = {
     "c2c_1200": dict(kind="c2c", n=1200, batch=256, seed=1, dtype="float"),
