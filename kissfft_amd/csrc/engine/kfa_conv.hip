@@ -7,8 +7,10 @@
 
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../kfa_engine.h"
@@ -19,51 +21,44 @@ namespace kfa {
 
 struct ConvPlan {
     const ConvVariant *v = nullptr;
-    void *stw_f = nullptr, *stw_i = nullptr, *H = nullptr;
+    DevBuf mem_f, mem_i, mem_h;
 };
 
 static std::mutex g_conv_mu;
-static std::map<std::tuple<int, int, int, std::string>, ConvPlan> g_conv;
+static std::map<std::tuple<int, int, int, std::string>, std::shared_ptr<const ConvPlan>> g_conv;
 
-static int conv_plan(int dev, int nfft, int real, const void *H, const ConvPlan **out)
+static int conv_plan(int dev, int nfft, int real, const void *H, std::shared_ptr<const ConvPlan> &out)
 {
     const std::string key_h((const char *)H, (size_t)nfft * 8);
     std::lock_guard<std::mutex> g(g_conv_mu);
     const auto key = std::make_tuple(dev, nfft, real, key_h);
     auto it = g_conv.find(key);
     if (it != g_conv.end()) {
-        *out = &it->second;
+        out = it->second;
         return 0;
     }
-    ConvPlan cp;
+    auto cp = std::make_shared<ConvPlan>();
     for (const ConvVariant &v : conv_registry())
         if (v.N == nfft && v.real == real) {  // the first registered of a length
-            cp.v = &v;
+            cp->v = &v;
             break;
         }
-    if (!cp.v)
+    if (!cp->v)
         return fail(-2, "fast convolution: no kernel for block length %d (powers of two 256..8192)", nfft);
     const std::vector<unsigned char> tf = float_twiddles(nfft, 0, nfft, 1), ti = float_twiddles(nfft, 1, nfft, 1);
     int rc;
-    if ((rc = make_stage_table_raw(tf.data(), KFA_F32, nfft, cp.v->radices, cp.v->tw_pad, 0, &cp.stw_f)) ||
-        (rc = make_stage_table_raw(ti.data(), KFA_F32, nfft, cp.v->radices, cp.v->tw_pad, 0, &cp.stw_i)))
+    if ((rc = make_stage_table_raw(tf.data(), KFA_F32, nfft, cp->v->radices, cp->v->tw_pad, 0, cp->mem_f)) ||
+        (rc = make_stage_table_raw(ti.data(), KFA_F32, nfft, cp->v->radices, cp->v->tw_pad, 0, cp->mem_i)) ||
+        (rc = dev_upload(H, (size_t)nfft * 8, cp->mem_h)))
         return rc;
-    if (hipMalloc(&cp.H, (size_t)nfft * 8) != hipSuccess ||
-        hipMemcpy(cp.H, H, (size_t)nfft * 8, hipMemcpyHostToDevice) != hipSuccess)
-        return fail(-3, "fast convolution: filter upload failed");
-    auto &slot = g_conv[key];
-    slot = cp;
-    *out = &slot;
+    g_conv[key] = cp;
+    out = cp;
     return 0;
 }
 
 void conv_release()
 {
     std::lock_guard<std::mutex> g(g_conv_mu);
-    for (auto &kv : g_conv)
-        for (void *p : {kv.second.stw_f, kv.second.stw_i, kv.second.H})
-            if (p)
-                (void)hipFree(p);
     g_conv.clear();
 }
 
@@ -83,8 +78,8 @@ extern "C" int kfa_fastfir(int type, int nfft, int real, const void *H, const vo
     int rc = call_begin(stream, &s, &dev);
     if (rc || nblocks == 0)
         return rc;
-    const ConvPlan *cp = nullptr;
-    if ((rc = conv_plan(dev, nfft, real, H, &cp)))
+    std::shared_ptr<const ConvPlan> cp;
+    if ((rc = conv_plan(dev, nfft, real, H, cp)))
         return rc;
     const size_t sb = real ? 4 : 8;
     Operand I, O;
@@ -95,9 +90,9 @@ extern "C" int kfa_fastfir(int type, int nfft, int real, const void *H, const vo
         ConvArgs a{};
         a.in = I.dev;
         a.out = O.dev;
-        a.stw_f = cp->stw_f;
-        a.stw_i = cp->stw_i;
-        a.H = cp->H;
+        a.stw_f = cp->mem_f.get();
+        a.stw_i = cp->mem_i.get();
+        a.H = cp->mem_h.get();
         a.nblocks = nblocks;
         a.nlines = real ? (nblocks + 1) / 2 : nblocks;
         a.ngood = ngood;

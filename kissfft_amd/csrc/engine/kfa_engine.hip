@@ -36,6 +36,7 @@
 #include "kfa_tile.hpp"
 #include "kfa_tile_real.hpp"
 #include "kfa_plane.hpp"
+#include "kfa_dft.hpp"
 
 namespace kfa {
 
@@ -108,6 +109,143 @@ static void scratch_trim()
         (void)hipMemPoolTrimTo(kv.second, 0);
 }
 
+// ---- shared device memory: deferred release ----------------------------------------------
+static std::mutex g_defer_mu;
+static std::vector<void *> g_deferred;
+
+static void defer_free(void *p)
+{
+    if (!p)
+        return;
+    std::lock_guard<std::mutex> g(g_defer_mu);
+    g_deferred.push_back(p);
+}
+
+void free_deferred()
+{
+    std::vector<void *> v;
+    {
+        std::lock_guard<std::mutex> g(g_defer_mu);
+        v.swap(g_deferred);
+    }
+    for (void *p : v)
+        (void)hipFree(p);  // waits for the device before freeing
+}
+
+int dev_upload(const void *host, size_t bytes, DevBuf &out)
+{
+    if (thread_noalloc())
+        return noalloc_refuse("a device table");
+    void *p = nullptr;
+    KFA_HIP(hipMalloc(&p, bytes ? bytes : 1));
+    out = DevBuf(p, defer_free);
+    if (bytes && host)
+        KFA_HIP(hipMemcpy(p, host, bytes, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int dev_alloc(size_t bytes, DevBuf &out)
+{
+    return dev_upload(nullptr, bytes, out);
+}
+
+// ---- per-call scratch: workspace carving or the pool ------------------------------------
+Workspace *&thread_workspace()
+{
+    static thread_local Workspace *ws = nullptr;
+    return ws;
+}
+
+bool &thread_noalloc()
+{
+    static thread_local bool on = false;
+    return on;
+}
+
+int noalloc_refuse(const char *what)
+{
+    return fail(-2, "no-allocation call: %s would have to be allocated (run the same transform once through "
+                    "the allocating entry point first, with device buffers)", what);
+}
+
+int Scratch::get(size_t bytes, hipStream_t s)
+{
+    s_ = s;
+    if (Workspace *ws = thread_workspace()) {
+        const size_t off = (ws->used + 255) & ~(size_t)255;
+        if (off + bytes > ws->size)
+            return fail(-2, "scratch of %zu bytes does not fit the %zu-byte workspace (%zu in use)", bytes, ws->size,
+                        ws->used);
+        ws_ = ws;
+        mark_ = ws->used;
+        ws->used = off + bytes;
+        p = ws->base + off;
+        return 0;
+    }
+    if (thread_noalloc())
+        return noalloc_refuse("scratch memory");
+    KFA_HIP(scratch_alloc(&p, bytes, s));
+    pooled_ = true;
+    return 0;
+}
+
+Scratch::~Scratch()
+{
+    if (ws_)
+        ws_->used = mark_;
+    else if (pooled_ && p)
+        (void)hipFreeAsync(p, s_);
+}
+
+static size_t chunk_limit()
+{
+    static const size_t lim = [] {
+        const char *v = getenv("KISSFFT_AMD_CHUNK_MB");
+        const long long mb = v && *v ? atoll(v) : 0;
+        return mb > 0 ? (size_t)mb << 20 : kScratchChunk;
+    }();
+    return lim;
+}
+
+size_t scratch_room()
+{
+    if (Workspace *ws = thread_workspace()) {
+        const size_t off = (ws->used + 255) & ~(size_t)255;
+        // room for a few aligned carvings of one chunk
+        return off + 1024 < ws->size ? ws->size - off - 1024 : 0;
+    }
+    return chunk_limit();
+}
+
+// ---- tuning stream ---------------------------------------------------------------------------
+static std::mutex g_tune_mu;
+static std::map<int, hipStream_t> g_tune_streams;
+
+int tune_stream(hipStream_t *s)
+{
+    int dev = 0;
+    KFA_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tune_streams.find(dev);
+    if (it == g_tune_streams.end()) {
+        hipStream_t t = nullptr;
+        KFA_HIP(hipStreamCreateWithFlags(&t, hipStreamNonBlocking));  // default priority
+        it = g_tune_streams.emplace(dev, t).first;
+    }
+    *s = it->second;
+    return 0;
+}
+
+bool stream_capturing(hipStream_t s)
+{
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &st) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return st != hipStreamCaptureStatusNone;
+}
+
 // largest line kept whole in LDS (two ping-pong buffers must fit 160 KiB)
 int line_cap(int type) { return type == KFA_Q15 ? 16384 : 8192; }
 
@@ -167,6 +305,8 @@ int build_cplan(const kfa_cplan *hp, int dev, std::shared_ptr<DevPlan> &out)
             return 0;
         }
     }
+    if (thread_noalloc())
+        return noalloc_refuse("the device plan");
     auto P = std::make_shared<DevPlan>();
     P->type = hp->type;
     P->n = hp->nfft;
@@ -200,8 +340,16 @@ int build_cplan(const kfa_cplan *hp, int dev, std::shared_ptr<DevPlan> &out)
                 }
             }
         }
+        if (best < 0 && P->radix[0] > cap && (long long)n / P->radix[0] <= cap) {
+            // kf_factor takes a cofactor above sqrt(n) whole, as the last factor: it runs
+            // first (m = 1), as one DFT of radix[0] points per column.  That stage runs as
+            // a global-memory pass (k_dft_stage), the remaining stages as the second pass.
+            best = 1;
+            P->bigp = true;
+        }
         if (best < 0)
-            return fail(-2, "nfft=%d: a radix factor exceeds the %d-point LDS line; not supported", n, cap);
+            return fail(-2, "nfft=%d: radix factors exceed the %d-point LDS line in more than the first stage; "
+                            "not supported", n, cap);
         long long l1 = 1;
         for (int j = 0; j < best; ++j)
             l1 *= P->radix[j];
@@ -209,20 +357,23 @@ int build_cplan(const kfa_cplan *hp, int dev, std::shared_ptr<DevPlan> &out)
         P->L1 = (int)l1;
         P->Q = (int)(n / l1);
         // pass 1: lines are the Q decimated columns, local fstride P/Q
-        fill_stages(P->lp1, P->radix, 0, best, n, P->L1, P->Q, 1, 1, 0, P->inverse);
+        if (!P->bigp)
+            fill_stages(P->lp1, P->radix, 0, best, n, P->L1, P->Q, 1, 1, 0, P->inverse);
         // pass 2: lines are the L1 rows of length Q, column index u = lo + L1*u_local
         fill_stages(P->lp2, P->radix, best, K, n, P->Q, 1, P->L1, P->L1, 1, P->inverse);
     }
     const size_t tw_bytes = cpx_bytes(hp->type) * (size_t)n;
     P->h_tw.assign((const unsigned char *)hp->twiddles, (const unsigned char *)hp->twiddles + tw_bytes);
-    KFA_HIP(hipMalloc(&P->d_tw, tw_bytes));
-    KFA_HIP(hipMemcpy(P->d_tw, hp->twiddles, tw_bytes, hipMemcpyHostToDevice));
+    {
+        RelaxCapture relax;
+        if (int rc = dev_upload(hp->twiddles, tw_bytes, P->tw_mem))
+            return rc;
+    }
+    P->d_tw = P->tw_mem.get();
     ++g_plans_built;
     std::lock_guard<std::mutex> g(C.mu);
     auto it = C.cplans.find(key);
-    if (it != C.cplans.end()) {  // another thread won the race
-        (void)hipFree(P->d_tw);
-        P->d_tw = nullptr;
+    if (it != C.cplans.end()) {  // another thread won the race (this copy is released)
         out = it->second;
         return 0;
     }
@@ -246,14 +397,15 @@ int build_rtab(const kfa_rplan *hp, int dev, std::shared_ptr<RealTables> &out)
     }
     auto R = std::make_shared<RealTables>();
     const size_t cnt = (size_t)(hp->half.nfft / 2);
-    const size_t bytes = cpx_bytes(hp->half.type) * (cnt > 0 ? cnt : 1);
-    KFA_HIP(hipMalloc(&R->d_stw, bytes));
-    if (cnt)
-        KFA_HIP(hipMemcpy(R->d_stw, hp->super_twiddles, bytes, hipMemcpyHostToDevice));
+    {
+        RelaxCapture relax;
+        if (int rc = dev_upload(hp->super_twiddles, cpx_bytes(hp->half.type) * cnt, R->stw_mem))
+            return rc;
+    }
+    R->d_stw = R->stw_mem.get();
     std::lock_guard<std::mutex> g(C.mu);
     auto it = C.rtabs.find(key);
     if (it != C.rtabs.end()) {
-        (void)hipFree(R->d_stw);
         out = it->second;
         return 0;
     }
@@ -277,7 +429,7 @@ bool env_flag(const char *name, bool dflt)
 // holds table[r*u*P_s] for r = 1..p-1, padded to a multiple of `pad` entries; then
 // table[P_s*m_s] and table[2*P_s*m_s], padded likewise (the layout of Radices<G, ...>).
 int make_stage_table_raw(const unsigned char *h_tw, int type, long long n, const std::vector<int> &rad,
-                                int pad, int fmt, void **d_out)
+                                int pad, int fmt, DevBuf &out)
 {
     // fmt 1: a Q15 entry w becomes the word pair (w.r, -w.i), (w.i, w.r) (8 bytes)
     const size_t cb = fmt == 1 ? 8 : cpx_bytes(type);
@@ -319,14 +471,13 @@ int make_stage_table_raw(const unsigned char *h_tw, int type, long long n, const
         fill_to(c0 + rup(2, pad));
         m *= p;
     }
-    KFA_HIP(hipMalloc(d_out, h.size()));
-    KFA_HIP(hipMemcpy(*d_out, h.data(), h.size(), hipMemcpyHostToDevice));
-    return 0;
+    RelaxCapture relax;
+    return dev_upload(h.data(), h.size(), out);
 }
 
-static int make_stage_table(const DevPlan &P, const std::vector<int> &rad, int pad, int fmt, void **d_out)
+static int make_stage_table(const DevPlan &P, const std::vector<int> &rad, int pad, int fmt, DevBuf &out)
 {
-    return make_stage_table_raw(P.h_tw.data(), P.type, P.n, rad, pad, fmt, d_out);
+    return make_stage_table_raw(P.h_tw.data(), P.type, P.n, rad, pad, fmt, out);
 }
 
 // Float twiddles of length n, the host library's expression (kiss_fft.c:361-367): phase
@@ -348,22 +499,20 @@ std::vector<unsigned char> float_twiddles(long long n, int inverse, long long co
 // ------------------------------------------------------------------------------------
 // tile variants (float): choice per (length, direction, mapping), four-step twiddles
 // ------------------------------------------------------------------------------------
-struct TileChoice {
-    const TileVariant *v = nullptr;
-    void *d_stw = nullptr;
-};
-
 struct PlaneChoice {
     const PlaneVariant *v = nullptr;
+    DevBuf mem_a, mem_b;
     void *stw_a = nullptr, *stw_b = nullptr;
 };
 
+typedef std::pair<DevBuf, DevBuf> BufPair;
+
 struct TileCache {
     std::mutex mu;
-    std::map<std::tuple<int, int, int, int>, TileChoice> choice;            // dev, n, inv, mode
-    std::map<std::tuple<int, long long, int>, std::pair<void *, void *>> fs; // dev, N, inv -> hi, lo
-    std::map<std::tuple<int, long long, int, int>, std::pair<void *, void *>> rs;  // dev, nc, n, inv
-    std::map<std::tuple<int, int, int, int>, PlaneChoice> plane;                  // dev, A, B, inv
+    std::map<std::tuple<int, int, int, int>, std::shared_ptr<const TileChoice>> choice;   // dev, n, inv, mode
+    std::map<std::tuple<int, long long, int>, BufPair> fs;                                // dev, N, inv -> hi, lo
+    std::map<std::tuple<int, long long, int, int>, BufPair> rs;                           // dev, nc, n, inv
+    std::map<std::tuple<int, int, int, int>, std::shared_ptr<const PlaneChoice>> plane;  // dev, A, B, inv
 };
 
 static TileCache &tile_cache()
@@ -405,94 +554,136 @@ static int fill_tune(void *p, size_t bytes, int type, hipStream_t s)
 static int launch_tile(const TileChoice &tc, TileArgs a, hipStream_t s);
 static int launch_rtile(const TileChoice &tc, RealTileArgs a, long long howmany, hipStream_t s);
 
-// Times the tile candidates of one (length, direction, mapping) on a scratch layout of
-// that mapping (1024 lines per group, kTuneValues complex values per side) and returns
-// the index of the fastest; interleaved rounds as for the line kernels.  The fused real
-// mappings time transforms of n * 1024 complex values.
-// Tile and plane candidates are timed on 512 MiB per side: larger than the 256 MB
-// Infinity Cache, so the ranking is the one of HBM-resident batches (on a cache-resident
-// 128 MiB the ranking differed: a variant 18 % slower on a 4 GiB batch won).
-constexpr long long kTuneValues = 64LL << 20;
-
-static int tune_tiles(int n, int mode, std::vector<TileChoice> &cand, hipStream_t s, size_t *best)
+// Times candidates: `run(i)` enqueues candidate i on the tuning stream; returns the index
+// of the fastest (a later candidate must win by 4 %: registration order, fastest in
+// full-size sweeps first, breaks near-ties so the choice does not flip with timing noise).
+// Interleaved rounds, each candidate's best round counts: a clock or power transient during
+// one candidate's turn does not decide the choice.  Any failure returns candidate 0.
+template <class Run>
+static size_t time_candidates(size_t ncand, hipStream_t ts, Run run, const char *what)
 {
-    const long long C = 1024;
-    const long long outer = std::max(1LL, kTuneValues / ((long long)n * C));
-    const bool real = mode == TILE_RSPLIT || mode == TILE_RMERGE;
-    const size_t bytes = 8 * (size_t)(n * C + 1) * (size_t)outer;
-    void *bin = nullptr, *bout = nullptr, *whi = nullptr, *wlo = nullptr, *rtw = nullptr;
-    const size_t hib = 8 * (size_t)((long long)n * C / 1024 + 1);
-    const size_t rtb = 8 * (size_t)(n + C);
-    KFA_HIP(scratch_alloc(&bin, bytes, s));
-    KFA_HIP(scratch_alloc(&bout, bytes, s));
-    KFA_HIP(scratch_alloc(&whi, hib, s));
-    KFA_HIP(scratch_alloc(&wlo, 8 * 1024, s));
-    KFA_HIP(scratch_alloc(&rtw, rtb, s));
-    if (int rc0 = fill_tune(bin, bytes, KFA_F32, s))
-        return rc0;
-    KFA_HIP(hipMemsetAsync(whi, 0x3c, hib, s));
-    KFA_HIP(hipMemsetAsync(wlo, 0x3c, 8 * 1024, s));
-    KFA_HIP(hipMemsetAsync(rtw, 0x3c, rtb, s));
-    TileArgs a{};
-    a.in = bin;
-    a.out = bout;
-    a.tw_hi = whi;
-    a.tw_lo = wlo;
-    a.nlines = outer * C;
-    a.nlo = C;
-    a.in_hi = a.out_hi = (long long)n * C;
-    a.out_lo = 1;
-    a.out_el = C;
-    a.in_lo = mode == TILE_ROWS_TO_COLS ? n : 1;
-    a.in_el = mode == TILE_ROWS_TO_COLS ? 1 : C;
-    RealTileArgs ra{};
-    ra.in = bin;
-    ra.out = bout;
-    ra.rtw_a = rtw;
-    ra.rtw_b = (const char *)rtw + 8 * C;
-    ra.tw_hi = whi;
-    ra.tw_lo = wlo;
-    ra.nc = (long long)n * C;
-    ra.in_hi = ra.out_hi = ra.nc + 1;
-    ra.other = (int)C;
-    auto run = [&](size_t i) { return real ? launch_rtile(cand[i], ra, outer, s) : launch_tile(cand[i], a, s); };
-    hipEvent_t e0, e1;
-    KFA_HIP(hipEventCreate(&e0));
-    KFA_HIP(hipEventCreate(&e1));
-    std::vector<float> t(cand.size(), 1e30f);
-    int rc = 0;
-    for (size_t i = 0; i < cand.size() && rc == 0; ++i)
-        rc = run(i);  // warm-up
-    for (int round = 0; round < 3 && rc == 0; ++round)
-        for (size_t i = 0; i < cand.size() && rc == 0; ++i) {
-            KFA_HIP(hipEventRecord(e0, s));
-            for (int r = 0; r < 3 && rc == 0; ++r)
-                rc = run(i);
-            KFA_HIP(hipEventRecord(e1, s));
-            KFA_HIP(hipEventSynchronize(e1));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        (void)hipGetLastError();
+        if (e0)
+            (void)hipEventDestroy(e0);
+        return 0;
+    }
+    std::vector<float> t(ncand, 1e30f);
+    bool ok = true;
+    for (size_t i = 0; i < ncand && ok; ++i)
+        ok = run(i) == 0;  // warm-up (clocks, code objects)
+    for (int round = 0; round < 3 && ok; ++round)
+        for (size_t i = 0; i < ncand && ok; ++i) {
+            ok = hipEventRecord(e0, ts) == hipSuccess;
+            for (int r = 0; r < 3 && ok; ++r)
+                ok = run(i) == 0;
+            ok = ok && hipEventRecord(e1, ts) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
             float ms = 0;
-            KFA_HIP(hipEventElapsedTime(&ms, e0, e1));
-            t[i] = std::min(t[i], ms);
+            ok = ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+            if (ok)
+                t[i] = std::min(t[i], ms);
         }
-    *best = 0;
-    for (size_t i = 1; i < cand.size(); ++i)
-        if (t[i] < 0.96f * t[*best])
-            *best = i;
-    if (env_flag("KISSFFT_AMD_VERBOSE", false))
-        for (size_t i = 0; i < cand.size(); ++i)
-            fprintf(stderr, "[kissfft_amd]   %-32s mode %d %8.1f us\n", cand[i].v->name, mode, 1e3f * t[i] / 3);
+    (void)hipStreamSynchronize(ts);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    for (void *p : {bin, bout, whi, wlo, rtw})
-        (void)hipFreeAsync(p, s);
-    KFA_HIP(hipStreamSynchronize(s));
-    return rc;
+    size_t best = 0;
+    if (!ok) {
+        (void)hipGetLastError();
+        if (env_flag("KISSFFT_AMD_VERBOSE", false))
+            fprintf(stderr, "[kissfft_amd] %s: timing failed, taking the first registered variant\n", what);
+        return 0;
+    }
+    for (size_t i = 1; i < ncand; ++i)
+        if (t[i] < 0.96f * t[best])
+            best = i;
+    if (env_flag("KISSFFT_AMD_VERBOSE", false))
+        for (size_t i = 0; i < ncand; ++i)
+            fprintf(stderr, "[kissfft_amd]   %s candidate %zu %8.1f us\n", what, i, 1e3f * t[i] / 3);
+    return best;
 }
 
-// The tile variant for (n, inverse, mode) on the current device, or v == nullptr; several
-// candidates are timed once (KISSFFT_AMD_TILE forces one by name, KISSFFT_AMD_TUNE=0
-// takes the first registered).
-static int tile_choice(int n, int inverse, int mode, hipStream_t s, const TileChoice **out)
+// Scratch for timing on the tuning stream, returned to the pool when the object goes.
+struct TuneBuf {
+    void *p = nullptr;
+    hipStream_t s = nullptr;
+    TuneBuf(size_t bytes, hipStream_t ts) : s(ts)
+    {
+        if (scratch_alloc(&p, bytes, ts) != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+        }
+    }
+    ~TuneBuf()
+    {
+        if (p)
+            (void)hipFreeAsync(p, s);
+    }
+};
+
+// Times the tile candidates of one (length, direction, mapping) on a scratch layout of
+// that mapping (1024 lines per group, kTuneValues complex values per side) and returns
+// the index of the fastest.  The fused real mappings time transforms of n * 1024 complex
+// values.  Tile and plane candidates are timed on 512 MiB per side: larger than the 256 MB
+// Infinity Cache, so the ranking is the one of HBM-resident batches (on a cache-resident
+// 128 MiB the ranking differed: a variant 18 % slower on a 4 GiB batch won).  When the
+// buffers cannot be had the smaller sizes are tried, then the first candidate is taken.
+constexpr long long kTuneValues = 64LL << 20;
+
+static size_t tune_tiles(int n, int mode, const std::vector<std::shared_ptr<TileChoice>> &cand)
+{
+    hipStream_t ts;
+    if (tune_stream(&ts))
+        return 0;
+    const long long C = 1024;
+    const bool real = mode == TILE_RSPLIT || mode == TILE_RMERGE;
+    for (long long values = kTuneValues; values >= (kTuneValues >> 4); values >>= 2) {
+        const long long outer = std::max(1LL, values / ((long long)n * C));
+        const size_t bytes = 8 * (size_t)(n * C + 1) * (size_t)outer;
+        const size_t hib = 8 * (size_t)((long long)n * C / 1024 + 1);
+        const size_t rtb = 8 * (size_t)(n + C);
+        TuneBuf bin(bytes, ts), bout(bytes, ts), whi(hib, ts), wlo(8 * 1024, ts), rtw(rtb, ts);
+        if (!bin.p || !bout.p || !whi.p || !wlo.p || !rtw.p)
+            continue;
+        if (fill_tune(bin.p, bytes, KFA_F32, ts) || hipMemsetAsync(whi.p, 0x3c, hib, ts) != hipSuccess ||
+            hipMemsetAsync(wlo.p, 0x3c, 8 * 1024, ts) != hipSuccess || hipMemsetAsync(rtw.p, 0x3c, rtb, ts) != hipSuccess)
+            return 0;
+        TileArgs a{};
+        a.in = bin.p;
+        a.out = bout.p;
+        a.tw_hi = whi.p;
+        a.tw_lo = wlo.p;
+        a.nlines = outer * C;
+        a.nlo = C;
+        a.in_hi = a.out_hi = (long long)n * C;
+        a.out_lo = 1;
+        a.out_el = C;
+        a.in_lo = mode == TILE_ROWS_TO_COLS ? n : 1;
+        a.in_el = mode == TILE_ROWS_TO_COLS ? 1 : C;
+        RealTileArgs ra{};
+        ra.in = bin.p;
+        ra.out = bout.p;
+        ra.rtw_a = rtw.p;
+        ra.rtw_b = (const char *)rtw.p + 8 * C;
+        ra.tw_hi = whi.p;
+        ra.tw_lo = wlo.p;
+        ra.nc = (long long)n * C;
+        ra.in_hi = ra.out_hi = ra.nc + 1;
+        ra.other = (int)C;
+        char what[64];
+        snprintf(what, sizeof what, "tile n=%d mode=%d", n, mode);
+        return time_candidates(cand.size(), ts, [&](size_t i) {
+            return real ? launch_rtile(*cand[i], ra, outer, ts) : launch_tile(*cand[i], a, ts);
+        }, what);
+    }
+    return 0;
+}
+
+// The tile variant for (n, inverse, mode) on the current device (v == nullptr: none);
+// several candidates are timed once (KISSFFT_AMD_TILE forces one by name,
+// KISSFFT_AMD_TUNE=0 takes the first registered; while the caller's stream `s` is being
+// captured the first registered is taken untimed).
+static int tile_choice(int n, int inverse, int mode, hipStream_t s, std::shared_ptr<const TileChoice> &out)
 {
     int dev;
     int rc = current_device(&dev);
@@ -503,46 +694,43 @@ static int tile_choice(int n, int inverse, int mode, hipStream_t s, const TileCh
     const auto key = std::make_tuple(dev, n, inverse, mode);
     auto it = C.choice.find(key);
     if (it != C.choice.end()) {
-        *out = &it->second;
+        out = it->second;
         return 0;
     }
-    std::vector<TileChoice> cand;
+    if (thread_noalloc())
+        return noalloc_refuse("a tile kernel's tables");
+    std::vector<std::shared_ptr<TileChoice>> cand;
     if (env_flag("KISSFFT_AMD_FAST", true)) {
         const char *force = getenv("KISSFFT_AMD_TILE");
         for (const TileVariant &v : tile_registry())
             if (v.n == n && v.inverse == inverse && v.mode == mode && (!force || !*force || !strcmp(force, v.name)))
-                cand.push_back(TileChoice{&v, nullptr});
-        if (cand.empty() && force && *force)  // forced name of another length: all candidates
+                cand.push_back(std::make_shared<TileChoice>(TileChoice{&v, nullptr, nullptr}));
+        if (cand.empty() && force && *force)  // forced name of another length: the first candidate
             for (const TileVariant &v : tile_registry())
                 if (v.n == n && v.inverse == inverse && v.mode == mode) {
-                    cand.push_back(TileChoice{&v, nullptr});
+                    cand.push_back(std::make_shared<TileChoice>(TileChoice{&v, nullptr, nullptr}));
                     break;
                 }
     }
     const std::vector<unsigned char> tw = float_twiddles(n, inverse, n, 1);
-    for (TileChoice &c : cand)
-        if ((rc = make_stage_table_raw(tw.data(), KFA_F32, n, c.v->radices, c.v->tw_pad, 0, &c.d_stw)))
+    for (auto &c : cand) {
+        if ((rc = make_stage_table_raw(tw.data(), KFA_F32, n, c->v->radices, c->v->tw_pad, 0, c->stw_mem)))
             return rc;
-    size_t best = 0;
-    if (cand.size() > 1 && env_flag("KISSFFT_AMD_TUNE", true) && (rc = tune_tiles(n, mode, cand, s, &best)))
-        return rc;
-    TileChoice tc;
-    for (size_t i = 0; i < cand.size(); ++i) {
-        if (i == best)
-            tc = cand[i];
-        else
-            (void)hipFree(cand[i].d_stw);
+        c->d_stw = c->stw_mem.get();
     }
-    if (tc.v && env_flag("KISSFFT_AMD_VERBOSE", false))
-        fprintf(stderr, "[kissfft_amd] tile n=%d inverse=%d mode=%d: %s\n", n, inverse, mode, tc.v->name);
-    auto &slot = C.choice[key];
-    slot = tc;
-    *out = &slot;
+    size_t best = 0;
+    if (cand.size() > 1 && env_flag("KISSFFT_AMD_TUNE", true) && !stream_capturing(s))
+        best = tune_tiles(n, mode, cand);
+    std::shared_ptr<const TileChoice> tc = cand.empty() ? std::make_shared<const TileChoice>() : cand[best];
+    if (tc->v && env_flag("KISSFFT_AMD_VERBOSE", false))
+        fprintf(stderr, "[kissfft_amd] tile n=%d inverse=%d mode=%d: %s\n", n, inverse, mode, tc->v->name);
+    C.choice[key] = tc;
+    out = tc;
     return 0;
 }
 
 // W^(q*1024) for q <= N/1024 and W^r for r < 1024, W = exp(-+2*pi*i/N)
-static int fourstep_twiddles(long long N, int inverse, const void **hi, const void **lo)
+static int fourstep_twiddles(long long N, int inverse, BufPair &out)
 {
     int dev;
     int rc = current_device(&dev);
@@ -555,22 +743,20 @@ static int fourstep_twiddles(long long N, int inverse, const void **hi, const vo
     if (it == C.fs.end()) {
         const std::vector<unsigned char> h = float_twiddles(N, inverse, N / 1024 + 1, 1024);
         const std::vector<unsigned char> l = float_twiddles(N, inverse, 1024, 1);
-        void *dh = nullptr, *dl = nullptr;
-        KFA_HIP(hipMalloc(&dh, h.size()));
-        KFA_HIP(hipMalloc(&dl, l.size()));
-        KFA_HIP(hipMemcpy(dh, h.data(), h.size(), hipMemcpyHostToDevice));
-        KFA_HIP(hipMemcpy(dl, l.data(), l.size(), hipMemcpyHostToDevice));
-        it = C.fs.emplace(key, std::make_pair(dh, dl)).first;
+        BufPair bp;
+        RelaxCapture relax;
+        if ((rc = dev_upload(h.data(), h.size(), bp.first)) || (rc = dev_upload(l.data(), l.size(), bp.second)))
+            return rc;
+        it = C.fs.emplace(key, bp).first;
     }
-    *hi = it->second.first;
-    *lo = it->second.second;
+    out = it->second;
     return 0;
 }
 
 // The fused real passes' split / merge twiddles for half length nc, line length n and
 // other = nc / n lines: a[i] = exp(-+i*pi*(i/nc + 1/2)), i < other, and
 // b[e] = exp(-+i*pi*e/n), e < n (kiss_fftr.c:48-53 restated as a product).
-static int real_split_twiddles(long long nc, int n, int inverse, const void **ta, const void **tb)
+static int real_split_twiddles(long long nc, int n, int inverse, BufPair &out)
 {
     int dev;
     int rc = current_device(&dev);
@@ -595,15 +781,14 @@ static int real_split_twiddles(long long nc, int n, int inverse, const void **ta
             hb[2 * e] = (float)cos(ph);
             hb[2 * e + 1] = (float)sin(ph);
         }
-        void *da = nullptr, *db = nullptr;
-        KFA_HIP(hipMalloc(&da, ha.size() * 4));
-        KFA_HIP(hipMalloc(&db, hb.size() * 4));
-        KFA_HIP(hipMemcpy(da, ha.data(), ha.size() * 4, hipMemcpyHostToDevice));
-        KFA_HIP(hipMemcpy(db, hb.data(), hb.size() * 4, hipMemcpyHostToDevice));
-        it = C.rs.emplace(key, std::make_pair(da, db)).first;
+        BufPair bp;
+        RelaxCapture relax;
+        if ((rc = dev_upload(ha.data(), ha.size() * 4, bp.first)) ||
+            (rc = dev_upload(hb.data(), hb.size() * 4, bp.second)))
+            return rc;
+        it = C.rs.emplace(key, bp).first;
     }
-    *ta = it->second.first;
-    *tb = it->second.second;
+    out = it->second;
     return 0;
 }
 
@@ -649,11 +834,8 @@ static int launch_rtile(const TileChoice &tc, RealTileArgs a, long long howmany,
 // ------------------------------------------------------------------------------------
 struct BlueChoice {
     const BlueVariant *v = nullptr;
-    void *stw_f = nullptr, *stw_i = nullptr, *chirp = nullptr, *bfft = nullptr;
+    DevBuf mem_f, mem_i, mem_chirp, mem_bfft;
 };
-
-static std::mutex g_blue_mu;
-static std::map<std::tuple<int, int, int>, BlueChoice> g_blue;  // dev, n, inverse
 
 // In-place radix-2 FFT in FP64 (forward: exp(-2*pi*i*j*k/M)), M a power of two.
 static void fft_pow2_f64(std::vector<double> &re, std::vector<double> &im)
@@ -686,37 +868,33 @@ static void fft_pow2_f64(std::vector<double> &re, std::vector<double> &im)
     }
 }
 
-// The Bluestein variant and tables for a float plan, or v == nullptr when the plan has no
-// prime factor above 16 or no registered convolution length fits.
-static int blue_choice(const DevPlan &P, const BlueChoice **out)
+// The Bluestein variant and tables for a float plan (v == nullptr when the plan has no
+// prime factor above 16 or no registered convolution length fits), decided once per plan.
+static int blue_choice(DevPlan &P, std::shared_ptr<const BlueChoice> &out)
 {
-    static const BlueChoice none;
-    *out = &none;
-    if (P.type != KFA_F32 || !env_flag("KISSFFT_AMD_BLUESTEIN", true))
+    static const std::shared_ptr<const BlueChoice> none = std::make_shared<const BlueChoice>();
+    out = none;
+    if (P.type != KFA_F32 || !P.single || !env_flag("KISSFFT_AMD_BLUESTEIN", true))
         return 0;
     int pmax = 0;
     for (int p : P.radix)
         pmax = std::max(pmax, p);
     if (pmax <= 16)
         return 0;
-    int dev;
-    int rc = current_device(&dev);
-    if (rc)
-        return rc;
-    std::lock_guard<std::mutex> g(g_blue_mu);
-    const auto key = std::make_tuple(dev, P.n, P.inverse);
-    auto it = g_blue.find(key);
-    if (it != g_blue.end()) {
-        *out = &it->second;
+    std::lock_guard<std::mutex> g(P.mu);
+    if (P.blue_decided) {
+        out = P.blue;
         return 0;
     }
-    BlueChoice bc;
+    if (thread_noalloc())
+        return noalloc_refuse("Bluestein tables");
+    auto bc = std::make_shared<BlueChoice>();
     const long long n = P.n;
     for (const BlueVariant &v : blue_registry())
-        if (v.M >= 2 * n - 1 && (!bc.v || v.M < bc.v->M))
-            bc.v = &v;
-    if (bc.v) {
-        const int M = bc.v->M;
+        if (v.M >= 2 * n - 1 && (!bc->v || v.M < bc->v->M))
+            bc->v = &v;
+    if (bc->v) {
+        const int M = bc->v->M;
         const double pi = 3.14159265358979323846264338327;
         const double sg = P.inverse ? 1.0 : -1.0;
         std::vector<double> wr(n), wi(n);
@@ -745,67 +923,62 @@ static int blue_choice(const DevPlan &P, const BlueChoice **out)
             hb[2 * m + 1] = (float)(bi[m] / M);
         }
         const std::vector<unsigned char> tf = float_twiddles(M, 0, M, 1), ti = float_twiddles(M, 1, M, 1);
-        if ((rc = make_stage_table_raw(tf.data(), KFA_F32, M, bc.v->radices, bc.v->tw_pad, 0, &bc.stw_f)) ||
-            (rc = make_stage_table_raw(ti.data(), KFA_F32, M, bc.v->radices, bc.v->tw_pad, 0, &bc.stw_i)))
+        int rc;
+        RelaxCapture relax;
+        if ((rc = make_stage_table_raw(tf.data(), KFA_F32, M, bc->v->radices, bc->v->tw_pad, 0, bc->mem_f)) ||
+            (rc = make_stage_table_raw(ti.data(), KFA_F32, M, bc->v->radices, bc->v->tw_pad, 0, bc->mem_i)) ||
+            (rc = dev_upload(hw.data(), hw.size() * 4, bc->mem_chirp)) ||
+            (rc = dev_upload(hb.data(), hb.size() * 4, bc->mem_bfft)))
             return rc;
-        KFA_HIP(hipMalloc(&bc.chirp, hw.size() * 4));
-        KFA_HIP(hipMalloc(&bc.bfft, hb.size() * 4));
-        KFA_HIP(hipMemcpy(bc.chirp, hw.data(), hw.size() * 4, hipMemcpyHostToDevice));
-        KFA_HIP(hipMemcpy(bc.bfft, hb.data(), hb.size() * 4, hipMemcpyHostToDevice));
         if (env_flag("KISSFFT_AMD_VERBOSE", false))
-            fprintf(stderr, "[kissfft_amd] n=%lld inverse=%d: Bluestein %s\n", n, P.inverse, bc.v->name);
+            fprintf(stderr, "[kissfft_amd] n=%lld inverse=%d: Bluestein %s\n", n, P.inverse, bc->v->name);
     }
-    auto &slot = g_blue[key];
-    slot = bc;
-    *out = &slot;
+    P.blue = bc;
+    P.blue_decided = true;
+    out = bc;
     return 0;
+}
+
+static void blue_args(const BlueChoice &bc, int n, BlueArgs &b)
+{
+    b.stw_f = bc.mem_f.get();
+    b.stw_i = bc.mem_i.get();
+    b.chirp = bc.mem_chirp.get();
+    b.bfft = bc.mem_bfft.get();
+    b.n = n;
 }
 
 // Whether float plan P runs Bluestein's kernel; its tables are built now rather than at
 // the first launch (the multi-batch decides at creation, before any capture).
-int blue_prepare(const DevPlan &P, bool *applies)
+int blue_prepare(DevPlan &P, bool *applies, std::shared_ptr<const BlueChoice> *keep)
 {
-    const BlueChoice *bc = nullptr;
-    const int rc = blue_choice(P, &bc);
+    std::shared_ptr<const BlueChoice> bc;
+    const int rc = blue_choice(P, bc);
     *applies = rc == 0 && bc->v != nullptr;
+    if (keep)
+        *keep = bc;
     return rc;
 }
 
 // Bluestein launch for float plan P with b's buffers / segments / distances filled in by
-// the caller (the multi-batch's grouped items); the tables come from P's cached choice.
-int blue_launch(const DevPlan &P, BlueArgs &b, hipStream_t s)
+// the caller (the multi-batch's grouped items); the tables come from P's choice.
+int blue_launch(DevPlan &P, BlueArgs &b, hipStream_t s)
 {
-    const BlueChoice *bc = nullptr;
-    int rc = blue_choice(P, &bc);
+    std::shared_ptr<const BlueChoice> bc;
+    int rc = blue_choice(P, bc);
     if (rc)
         return rc;
     if (!bc->v)
         return fail(-2, "no Bluestein kernel for n=%d", P.n);
-    b.stw_f = bc->stw_f;
-    b.stw_i = bc->stw_i;
-    b.chirp = bc->chirp;
-    b.bfft = bc->bfft;
-    b.n = P.n;
+    blue_args(*bc, P.n, b);
     ++g_launches;
     return bc->v->launch(b, s) ? fail(-3, "Bluestein kernel launch failed") : 0;
-}
-
-static void blue_release()
-{
-    std::lock_guard<std::mutex> g(g_blue_mu);
-    for (auto &kv : g_blue) {
-        BlueChoice &b = kv.second;
-        for (void *p : {b.stw_f, b.stw_i, b.chirp, b.bfft})
-            if (p)
-                (void)hipFree(p);
-    }
-    g_blue.clear();
 }
 
 // The plane variant for the last two axes A x B (float) on the current device, or
 // v == nullptr.  Several candidates are timed once on 512 MiB of planes
 // (KISSFFT_AMD_PLANE=0 turns the plane pass off, KISSFFT_AMD_TILE forces a variant by name).
-static int plane_choice(int A, int B, int inverse, hipStream_t s, const PlaneChoice **out)
+static int plane_choice(int A, int B, int inverse, hipStream_t s, std::shared_ptr<const PlaneChoice> &out)
 {
     int dev;
     int rc = current_device(&dev);
@@ -816,97 +989,79 @@ static int plane_choice(int A, int B, int inverse, hipStream_t s, const PlaneCho
     const auto key = std::make_tuple(dev, A, B, inverse);
     auto it = C.plane.find(key);
     if (it != C.plane.end()) {
-        *out = &it->second;
+        out = it->second;
         return 0;
     }
-    std::vector<PlaneChoice> cand;
+    if (thread_noalloc())
+        return noalloc_refuse("a plane kernel's tables");
+    std::vector<std::shared_ptr<PlaneChoice>> cand;
     if (env_flag("KISSFFT_AMD_FAST", true) && env_flag("KISSFFT_AMD_PLANE", true)) {
         const char *force = getenv("KISSFFT_AMD_TILE");
         bool named = false;
         for (const PlaneVariant &v : plane_registry())
             named |= force && *force && !strcmp(force, v.name) && v.A == A && v.B == B;
         for (const PlaneVariant &v : plane_registry())
-            if (v.A == A && v.B == B && v.inverse == inverse && (!named || !strcmp(force, v.name)))
-                cand.push_back(PlaneChoice{&v, nullptr, nullptr});
+            if (v.A == A && v.B == B && v.inverse == inverse && (!named || !strcmp(force, v.name))) {
+                auto c = std::make_shared<PlaneChoice>();
+                c->v = &v;
+                cand.push_back(c);
+            }
     }
-    for (PlaneChoice &c : cand) {
+    for (auto &c : cand) {
         const std::vector<unsigned char> ta = float_twiddles(A, inverse, A, 1), tb = float_twiddles(B, inverse, B, 1);
-        if ((rc = make_stage_table_raw(ta.data(), KFA_F32, A, c.v->rad_a, c.v->tw_pad, 0, &c.stw_a)) ||
-            (rc = make_stage_table_raw(tb.data(), KFA_F32, B, c.v->rad_b, c.v->tw_pad, 0, &c.stw_b)))
+        if ((rc = make_stage_table_raw(ta.data(), KFA_F32, A, c->v->rad_a, c->v->tw_pad, 0, c->mem_a)) ||
+            (rc = make_stage_table_raw(tb.data(), KFA_F32, B, c->v->rad_b, c->v->tw_pad, 0, c->mem_b)))
             return rc;
+        c->stw_a = c->mem_a.get();
+        c->stw_b = c->mem_b.get();
     }
     size_t best = 0;
-    if (cand.size() > 1 && env_flag("KISSFFT_AMD_TUNE", true)) {
-        const long long planes = std::max(1LL, kTuneValues / ((long long)A * B));
-        const size_t bytes = 8 * (size_t)A * B * (size_t)planes;
-        void *bin = nullptr, *bout = nullptr;
-        KFA_HIP(scratch_alloc(&bin, bytes, s));
-        KFA_HIP(scratch_alloc(&bout, bytes, s));
-        if ((rc = fill_tune(bin, bytes, KFA_F32, s)))
-            return rc;
-        hipEvent_t e0, e1;
-        KFA_HIP(hipEventCreate(&e0));
-        KFA_HIP(hipEventCreate(&e1));
-        std::vector<float> t(cand.size(), 1e30f);
-        auto run = [&](size_t i) {
-            PlaneArgs pa{bin, bout, cand[i].stw_a, cand[i].stw_b, planes};
-            return cand[i].v->launch(pa, s);
-        };
-        for (size_t i = 0; i < cand.size() && rc == 0; ++i)
-            rc = run(i);
-        for (int round = 0; round < 3 && rc == 0; ++round)
-            for (size_t i = 0; i < cand.size() && rc == 0; ++i) {
-                KFA_HIP(hipEventRecord(e0, s));
-                for (int r = 0; r < 3 && rc == 0; ++r)
-                    rc = run(i);
-                KFA_HIP(hipEventRecord(e1, s));
-                KFA_HIP(hipEventSynchronize(e1));
-                float ms = 0;
-                KFA_HIP(hipEventElapsedTime(&ms, e0, e1));
-                t[i] = std::min(t[i], ms);
-            }
-        for (size_t i = 1; i < cand.size(); ++i)
-            if (t[i] < 0.96f * t[best])
-                best = i;
-        if (env_flag("KISSFFT_AMD_VERBOSE", false))
-            for (size_t i = 0; i < cand.size(); ++i)
-                fprintf(stderr, "[kissfft_amd]   %-32s plane %8.1f us\n", cand[i].v->name, 1e3f * t[i] / 3);
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        (void)hipFreeAsync(bin, s);
-        (void)hipFreeAsync(bout, s);
-        KFA_HIP(hipStreamSynchronize(s));
-        if (rc)
-            return fail(-3, "plane kernel launch failed");
-    }
-    PlaneChoice pc;
-    for (size_t i = 0; i < cand.size(); ++i) {
-        if (i == best) {
-            pc = cand[i];
-        } else {
-            (void)hipFree(cand[i].stw_a);
-            (void)hipFree(cand[i].stw_b);
+    hipStream_t ts;
+    if (cand.size() > 1 && env_flag("KISSFFT_AMD_TUNE", true) && !stream_capturing(s) && tune_stream(&ts) == 0) {
+        for (long long values = kTuneValues; values >= (kTuneValues >> 4); values >>= 2) {
+            const long long planes = std::max(1LL, values / ((long long)A * B));
+            const size_t bytes = 8 * (size_t)A * B * (size_t)planes;
+            TuneBuf bin(bytes, ts), bout(bytes, ts);
+            if (!bin.p || !bout.p)
+                continue;
+            if (fill_tune(bin.p, bytes, KFA_F32, ts))
+                break;
+            char what[64];
+            snprintf(what, sizeof what, "plane %dx%d", A, B);
+            best = time_candidates(cand.size(), ts, [&](size_t i) {
+                PlaneArgs pa{bin.p, bout.p, cand[i]->stw_a, cand[i]->stw_b, planes};
+                return cand[i]->v->launch(pa, ts);
+            }, what);
+            break;
         }
     }
-    if (pc.v && env_flag("KISSFFT_AMD_VERBOSE", false))
-        fprintf(stderr, "[kissfft_amd] plane %dx%d inverse=%d: %s\n", A, B, inverse, pc.v->name);
-    auto &slot = C.plane[key];
-    slot = pc;
-    *out = &slot;
+    std::shared_ptr<const PlaneChoice> pc = cand.empty() ? std::make_shared<const PlaneChoice>() : cand[best];
+    if (pc->v && env_flag("KISSFFT_AMD_VERBOSE", false))
+        fprintf(stderr, "[kissfft_amd] plane %dx%d inverse=%d: %s\n", A, B, inverse, pc->v->name);
+    C.plane[key] = pc;
+    out = pc;
     return 0;
 }
 
-// Float four-step (Bailey) for n = N1*N2 with both halves registered as tile variants:
-// pass 1 transforms the N1 columns (length N2, stride N1) and applies W_N^(n1*k2); pass 2
-// transforms the N2 rows (length N1) and writes them as columns (k = k2 + N2*k1).
-// Returns 1 (nothing done) when no such split exists.
-static int run_c2c_bailey(DevPlan &P, const void *in, long long istride, long long idist, void *out,
-                          long long ostride, long long odist, long long howmany, hipStream_t s)
+// Float four-step (Bailey) for n = N1*N2 with both halves registered as tile variants,
+// decided once per plan: pass 1 transforms the N1 columns (length N2, stride N1) and
+// applies W_N^(n1*k2); pass 2 transforms the N2 rows (length N1) and writes them as
+// columns (k = k2 + N2*k1).  P.bailey stays null when no such split exists.
+static int bailey_plan(DevPlan &P, hipStream_t s, std::shared_ptr<const BaileyPlan> &out)
 {
-    if (P.type != KFA_F32 || !env_flag("KISSFFT_AMD_FAST", true))
-        return 1;
+    std::lock_guard<std::mutex> g(P.mu);
+    if (P.bailey_decided) {
+        out = P.bailey;
+        return 0;
+    }
+    out = nullptr;
+    if (P.type != KFA_F32 || P.single || P.bigp || !env_flag("KISSFFT_AMD_FAST", true)) {
+        P.bailey_decided = true;
+        return 0;
+    }
+    if (thread_noalloc())
+        return noalloc_refuse("the four-step plan");
     const long long n = P.n;
-    const TileChoice *c1 = nullptr, *c2 = nullptr;
     long long best1 = 0;
     for (const TileVariant &v : tile_registry()) {
         if (v.mode != TILE_ROWS_TO_COLS || v.inverse != P.inverse || n % v.n != 0)
@@ -922,23 +1077,39 @@ static int run_c2c_bailey(DevPlan &P, const void *in, long long istride, long lo
         if (best1 == 0 || score(N1) < score(best1))
             best1 = N1;
     }
-    if (best1 == 0)
-        return 1;
-    const long long N1 = best1, N2 = n / N1;
-    int rc;
-    if ((rc = tile_choice((int)N1, P.inverse, TILE_ROWS_TO_COLS, s, &c2)) ||
-        (rc = tile_choice((int)N2, P.inverse, TILE_COLS_TWID, s, &c1)))
-        return rc;
-    if (!c1->v || !c2->v)
-        return 1;
+    if (best1) {
+        auto bp = std::make_shared<BaileyPlan>();
+        bp->N1 = best1;
+        bp->N2 = n / best1;
+        BufPair tw;
+        int rc;
+        if ((rc = tile_choice((int)bp->N1, P.inverse, TILE_ROWS_TO_COLS, s, bp->c2)) ||
+            (rc = tile_choice((int)bp->N2, P.inverse, TILE_COLS_TWID, s, bp->c1)) ||
+            (rc = fourstep_twiddles(n, P.inverse, tw)))
+            return rc;
+        bp->tw_hi = tw.first;
+        bp->tw_lo = tw.second;
+        if (bp->c1->v && bp->c2->v)
+            P.bailey = bp;
+    }
+    P.bailey_decided = true;
+    out = P.bailey;
+    return 0;
+}
+
+static int run_c2c_bailey(const BaileyPlan &bp, long long n, const void *in, long long istride, long long idist,
+                          void *out, long long ostride, long long odist, long long howmany, hipStream_t s)
+{
+    const long long N1 = bp.N1, N2 = bp.N2;
     TileArgs a{};
-    if ((rc = fourstep_twiddles(n, P.inverse, &a.tw_hi, &a.tw_lo)))
+    a.tw_hi = bp.tw_hi.get();
+    a.tw_lo = bp.tw_lo.get();
+    Scratch tmp;
+    if (int rc = tmp.get(8 * (size_t)n * (size_t)howmany, s))
         return rc;
-    void *tmp = nullptr;
-    KFA_HIP(scratch_alloc(&tmp, 8 * (size_t)n * (size_t)howmany, s));
     // pass 1: line (b, n1): element n2 at in[b*idist + (n1 + N1*n2)*istride] -> tmp
     a.in = in;
-    a.out = tmp;
+    a.out = tmp.p;
     a.nlines = howmany * N1;
     a.nlo = N1;
     a.in_hi = idist;
@@ -947,11 +1118,11 @@ static int run_c2c_bailey(DevPlan &P, const void *in, long long istride, long lo
     a.out_hi = n;
     a.out_lo = 1;
     a.out_el = N1;
-    rc = launch_tile(*c1, a, s);
+    int rc = launch_tile(*bp.c1, a, s);
     if (rc == 0) {
         // pass 2: line (b, k2): element n1 at tmp[b*n + n1 + N1*k2] -> out[k2 + N2*k1]
         TileArgs b{};
-        b.in = tmp;
+        b.in = tmp.p;
         b.out = out;
         b.nlines = howmany * N2;
         b.nlo = N2;
@@ -961,13 +1132,16 @@ static int run_c2c_bailey(DevPlan &P, const void *in, long long istride, long lo
         b.out_hi = odist;
         b.out_lo = ostride;
         b.out_el = N2 * ostride;
-        rc = launch_tile(*c2, b, s);
+        rc = launch_tile(*bp.c2, b, s);
     }
-    (void)hipFreeAsync(tmp, s);
     return rc;
 }
 
-// Chooses (once per plan and kind) the fastest registered variant, or none.
+// Chooses (once per plan and kind) the fastest registered variant, or none.  Timing runs
+// on the device's tuning stream with its own scratch; when that scratch cannot be had,
+// smaller batches are tried and then registration order decides.  While the caller's
+// stream is being captured the first registered variant is taken untimed (the capture is
+// not disturbed).  The choice is recorded only once its table exists.
 int choose_fast(DevPlan &P, int kind, hipStream_t s, const FastChoice **out, long long lines_hint)
 {
     std::lock_guard<std::mutex> g(P.mu);
@@ -975,9 +1149,10 @@ int choose_fast(DevPlan &P, int kind, hipStream_t s, const FastChoice **out, lon
     *out = &fc;
     if (fc.decided)
         return 0;
-    fc.decided = true;
-    if (!P.single || !env_flag("KISSFFT_AMD_FAST", true))
+    if (!P.single || !env_flag("KISSFFT_AMD_FAST", true)) {
+        fc.decided = true;
         return 0;
+    }
     std::vector<const FastVariant *> cand;
     for (const FastVariant &v : fast_registry()) {
         if (v.type != P.type || v.kind != kind || v.n != P.n || v.inverse != P.inverse)
@@ -986,14 +1161,17 @@ int choose_fast(DevPlan &P, int kind, hipStream_t s, const FastChoice **out, lon
             continue;  // fixed point must keep the reference schedule
         cand.push_back(&v);
     }
-    if (cand.empty())
+    if (cand.empty()) {
+        fc.decided = true;
         return 0;
-    std::vector<void *> tabs(cand.size(), nullptr);
+    }
+    if (thread_noalloc())
+        return noalloc_refuse("the kernel choice");
+    std::vector<DevBuf> tabs(cand.size());
     for (size_t i = 0; i < cand.size(); ++i) {
         if ((long long)(cand[i]->tw_len) <= 0)
             return fail(-2, "bad fast variant table");
-        int rc = make_stage_table(P, cand[i]->radices, cand[i]->tw_pad, cand[i]->tw_fmt, &tabs[i]);
-        if (rc)
+        if (int rc = make_stage_table(P, cand[i]->radices, cand[i]->tw_pad, cand[i]->tw_fmt, tabs[i]))
             return rc;
     }
     size_t best = 0;
@@ -1006,77 +1184,49 @@ int choose_fast(DevPlan &P, int kind, hipStream_t s, const FastChoice **out, lon
                 forced = true;
             }
     }
-    if (!forced && cand.size() > 1 && env_flag("KISSFFT_AMD_TUNE", true)) {
-        // time every candidate on a scratch batch large enough to stream from HBM
-        // as large as the call that asks (between 256 MiB and 1 GiB per side, beyond the
-        // last-level cache): the ranking of variants depends on the batch size
+    hipStream_t ts;
+    if (!forced && cand.size() > 1 && env_flag("KISSFFT_AMD_TUNE", true) && !stream_capturing(s) &&
+        tune_stream(&ts) == 0) {
+        // time every candidate on a scratch batch large enough to stream from HBM, as large
+        // as the call that asks (between 256 MiB and 1 GiB per side, beyond the last-level
+        // cache): the ranking of variants depends on the batch size
         const size_t cb = cpx_bytes(P.type);
         const long long per_line = (long long)cb * (P.n + 1);
         long long lines = std::max(lines_hint, (256LL << 20) / per_line);
         lines = std::max(64LL, std::min(lines, (1LL << 30) / per_line));
-        const size_t bytes = (size_t)per_line * (size_t)lines;
-        void *bin = nullptr, *bout = nullptr, *brtw = nullptr;
-        KFA_HIP(scratch_alloc(&bin, bytes, s));
-        KFA_HIP(scratch_alloc(&bout, bytes, s));
-        KFA_HIP(scratch_alloc(&brtw, cb * (size_t)(P.n / 2 + 1), s));
-        // random-looking data of the plan's kind: representative power draw and clock
-        if (int rc0 = fill_tune(bin, bytes, P.type, s))
-            return rc0;
-        KFA_HIP(hipMemsetAsync(brtw, 0x3c, cb * (size_t)(P.n / 2 + 1), s));
-        hipEvent_t e0, e1;
-        KFA_HIP(hipEventCreate(&e0));
-        KFA_HIP(hipEventCreate(&e1));
-        std::vector<float> t_best(cand.size(), 1e30f);
-        auto args_for = [&](size_t i) {
-            FastArgs a{};
-            a.in = bin;
-            a.out = bout;
-            a.stw = tabs[i];
-            a.rtw = brtw;
-            a.nlines = lines;
-            a.in_hi = kind == FK_C2R ? P.n + 1 : P.n;
-            a.in_el = 1;
-            a.out_hi = kind == FK_R2C ? P.n + 1 : P.n;
-            a.out_el = 1;
-            return a;
-        };
-        for (size_t i = 0; i < cand.size(); ++i)
-            cand[i]->launch(args_for(i), s);  // warm-up (clocks, code objects)
-        // interleaved rounds, each candidate's best round counts: a clock or power
-        // transient during one candidate's turn does not decide the choice
-        for (int round = 0; round < 3; ++round) {
-            for (size_t i = 0; i < cand.size(); ++i) {
-                const FastArgs a = args_for(i);
-                KFA_HIP(hipEventRecord(e0, s));
-                for (int r = 0; r < 3; ++r)
-                    cand[i]->launch(a, s);
-                KFA_HIP(hipEventRecord(e1, s));
-                KFA_HIP(hipEventSynchronize(e1));
-                float ms = 0;
-                KFA_HIP(hipEventElapsedTime(&ms, e0, e1));
-                t_best[i] = std::min(t_best[i], ms);
-            }
+        for (; lines >= 64; lines /= 4) {
+            const size_t bytes = (size_t)per_line * (size_t)lines;
+            const size_t rtb = cb * (size_t)(P.n / 2 + 1);
+            TuneBuf bin(bytes, ts), bout(bytes, ts), brtw(rtb, ts);
+            if (!bin.p || !bout.p || !brtw.p)
+                continue;
+            // random-looking data of the plan's kind: representative power draw and clock
+            if (fill_tune(bin.p, bytes, P.type, ts) || hipMemsetAsync(brtw.p, 0x3c, rtb, ts) != hipSuccess)
+                break;
+            auto args_for = [&](size_t i) {
+                FastArgs a{};
+                a.in = bin.p;
+                a.out = bout.p;
+                a.stw = tabs[i].get();
+                a.rtw = brtw.p;
+                a.nlines = lines;
+                a.in_hi = kind == FK_C2R ? P.n + 1 : P.n;
+                a.in_el = 1;
+                a.out_hi = kind == FK_R2C ? P.n + 1 : P.n;
+                a.out_el = 1;
+                return a;
+            };
+            char what[64];
+            snprintf(what, sizeof what, "n=%d kind=%d", P.n, kind);
+            best = time_candidates(cand.size(), ts, [&](size_t i) { return cand[i]->launch(args_for(i), ts); },
+                                   what);
+            break;
         }
-        // a later candidate must win by 4%: registration order (fastest in full-size
-        // sweeps first) breaks near-ties, so the choice does not flip with timing noise
-        for (size_t i = 1; i < cand.size(); ++i)
-            if (t_best[i] < 0.96f * t_best[best])
-                best = i;
-        if (env_flag("KISSFFT_AMD_VERBOSE", false))
-            for (size_t i = 0; i < cand.size(); ++i)
-                fprintf(stderr, "[kissfft_amd]   %-32s %8.1f us\n", cand[i]->name, 1e3f * t_best[i] / 3);
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        (void)hipFreeAsync(bin, s);
-        (void)hipFreeAsync(bout, s);
-        (void)hipFreeAsync(brtw, s);
-        KFA_HIP(hipStreamSynchronize(s));
     }
-    for (size_t i = 0; i < cand.size(); ++i)
-        if (i != best)
-            (void)hipFree(tabs[i]);
     fc.v = cand[best];
-    fc.d_stw = tabs[best];
+    fc.stw_mem = tabs[best];
+    fc.d_stw = fc.stw_mem.get();
+    fc.decided = true;
     if (env_flag("KISSFFT_AMD_VERBOSE", false))
         fprintf(stderr, "[kissfft_amd] n=%d kind=%d inverse=%d: fast variant %s\n", P.n, kind, P.inverse,
                 fc.v->name);
@@ -1133,9 +1283,12 @@ int stage_in(const void *p, size_t bytes, hipStream_t s, Operand &op)
         op.dev = const_cast<void *>(p);
         return 0;
     }
-    op.staged = true;
-    op.host = p;
+    if (thread_noalloc())
+        return noalloc_refuse("a staging copy of a host buffer");
     KFA_HIP(scratch_alloc(&op.dev, bytes ? bytes : 1, s));
+    op.staged = true;
+    op.s = s;
+    op.host = p;
     if (bytes) {
         KFA_HIP(hipMemcpyAsync(op.dev, p, bytes, hipMemcpyHostToDevice, s));
         g_h2d += (long long)bytes;
@@ -1150,9 +1303,12 @@ int stage_out(void *p, size_t bytes, hipStream_t s, Operand &op, bool gaps)
         op.dev = p;
         return 0;
     }
-    op.staged = true;
-    op.host = p;
+    if (thread_noalloc())
+        return noalloc_refuse("a staging copy of a host buffer");
     KFA_HIP(scratch_alloc(&op.dev, bytes ? bytes : 1, s));
+    op.staged = true;
+    op.s = s;
+    op.host = p;
     if (gaps && bytes) {  // the copy back covers the whole span
         KFA_HIP(hipMemcpyAsync(op.dev, p, bytes, hipMemcpyHostToDevice, s));
         g_h2d += (long long)bytes;
@@ -1175,6 +1331,7 @@ void release(Operand &op, hipStream_t s)
         (void)hipFreeAsync(op.dev, s);
     op.dev = nullptr;
 }
+
 
 // ------------------------------------------------------------------------------------
 // launches of the line kernel
@@ -1245,6 +1402,51 @@ static int launch_lines(int type, const LineLaunch &ll, hipStream_t s)
     }
 }
 
+// Whether device ranges [a, a+na) and [b, b+nb) overlap.
+static bool overlap(const void *a, size_t na, const void *b, size_t nb)
+{
+    const uintptr_t x = reinterpret_cast<uintptr_t>(a), y = reinterpret_cast<uintptr_t>(b);
+    return x < y + nb && y < x + na;
+}
+
+template <class A>
+static int launch_dft_stage_t(DftStageArgs a, hipStream_t s)
+{
+    a.qtiles = (a.p + 255) / 256;
+    const long long blocks = a.ncols * a.qtiles;
+    if (blocks <= 0)
+        return 0;
+    if (blocks > 0x7fffffffLL)
+        return fail(-2, "too many columns (%lld) for one DFT-stage launch", a.ncols);
+    hipLaunchKernelGGL(k_dft_stage<A>, dim3((unsigned)blocks), dim3(256), 0, s, a);
+    KFA_HIP(hipGetLastError());
+    ++g_launches;
+    return 0;
+}
+
+static int launch_dft_stage(int type, const DftStageArgs &a, hipStream_t s)
+{
+    switch (type) {
+    case KFA_F32: return launch_dft_stage_t<F32>(a, s);
+    case KFA_Q15: return launch_dft_stage_t<Q15>(a, s);
+    case KFA_Q31: return launch_dft_stage_t<Q31>(a, s);
+    default: return fail(-2, "unknown scalar kind %d", type);
+    }
+}
+
+size_t multipass_bytes_per_transform(DevPlan &P, int fk)
+{
+    if (P.single)
+        return 0;
+    const size_t cb = cpx_bytes(P.type);
+    const size_t c2c = (P.type == KFA_F32 ? 8 : cb) * (size_t)P.n + 256;  // the four-step temporary
+    if (fk == FK_C2C)
+        return c2c;
+    // real: the temporary of the fused passes or of the split / merge pass, plus the complex
+    // pass's own when the passes are not fused
+    return cb * (size_t)(P.n + 16) + 256 + c2c;
+}
+
 // Runs `howmany` complex transforms of plan P between two device buffers.
 // in: element t of item b at in + (b*idist + t*istride); out: out + (b*odist + t*ostride).
 int run_c2c(DevPlan &P, const void *in, long long istride, long long idist, void *out,
@@ -1255,9 +1457,13 @@ int run_c2c(DevPlan &P, const void *in, long long istride, long long idist, void
         return 0;
     if (!P.single) {
         // multi-pass lines use a temporary as large as their data: run large batches in
-        // chunks so the temporary stays within kScratchChunk bytes
+        // chunks whose temporary fits the scratch room (workspace or chunk limit)
         const size_t cb = cpx_bytes(P.type);
-        const long long chunk = std::max(1LL, (long long)(kScratchChunk / (cb * (size_t)n)));
+        const size_t per = multipass_bytes_per_transform(P, FK_C2C);
+        const size_t room = scratch_room();
+        if (room < per)
+            return fail(-2, "nfft=%d needs %zu bytes of scratch per transform; %zu available", n, per, room);
+        const long long chunk = std::max(1LL, (long long)(room / per));
         if (howmany > chunk) {
             for (long long b = 0; b < howmany; b += chunk) {
                 const int rc = run_c2c(P, (const char *)in + (size_t)(b * idist) * cb, istride, idist,
@@ -1287,18 +1493,14 @@ int run_c2c(DevPlan &P, const void *in, long long istride, long long idist, void
             ++g_launches;
             return fc->v->launch(a, s) ? fail(-3, "fast kernel launch failed") : 0;
         }
-        const BlueChoice *bc = nullptr;
-        if ((rc = blue_choice(P, &bc)))
+        std::shared_ptr<const BlueChoice> bc;
+        if ((rc = blue_choice(P, bc)))
             return rc;
         if (bc->v) {
             BlueArgs b{};
             b.in = in;
             b.out = out;
-            b.stw_f = bc->stw_f;
-            b.stw_i = bc->stw_i;
-            b.chirp = bc->chirp;
-            b.bfft = bc->bfft;
-            b.n = n;
+            blue_args(*bc, n, b);
             b.nlines = howmany;
             b.in_hi = idist;
             b.in_el = istride;
@@ -1324,33 +1526,76 @@ int run_c2c(DevPlan &P, const void *in, long long istride, long long idist, void
         return launch_lines(P.type, ll, s);
     }
     {
-        const int rc = run_c2c_bailey(P, in, istride, idist, out, ostride, odist, howmany, s);
-        if (rc <= 0)
+        std::shared_ptr<const BaileyPlan> bp;
+        if (int rc = bailey_plan(P, s, bp))
             return rc;
+        if (bp)
+            return run_c2c_bailey(*bp, n, in, istride, idist, out, ostride, odist, howmany, s);
     }
-    // four-step in the reference's stage order: pass 1 into a stream-ordered temporary,
-    // pass 2 into out
-    void *tmp = nullptr;
-    const size_t tbytes = cpx_bytes(P.type) * (size_t)n * (size_t)howmany;
-    KFA_HIP(scratch_alloc(&tmp, tbytes, s));
-    LineLaunch a{};
-    a.in = in;
-    a.out = tmp;
-    a.tw = P.d_tw;
-    a.pl = &P.lp1;
-    a.io.nlines = howmany * P.Q;
-    a.io.nlo = P.Q;
-    a.io.in_hi = idist;
-    a.io.in_lo = istride;
-    a.io.in_el = (long long)P.Q * istride;
-    a.io.out_hi = n;
-    a.io.out_lo = 1;
-    a.io.out_el = P.Q;
-    a.colmode = 1;
-    int rc = launch_lines(P.type, a, s);
+    // four-step in the reference's stage order: pass 1 into a temporary, pass 2 into out
+    DftStageArgs d{};
+    if (P.bigp) {
+        d.tw = P.d_tw;
+        d.n = n;
+        d.Q = P.Q;
+        d.p = P.L1;
+        d.ncols = howmany * P.Q;
+        d.in = in;
+        d.in_hi = idist;
+        d.in_el = istride;
+        if (P.Q == 1) {  // n is the prime itself: one pass
+            d.out = out;
+            d.out_hi = odist;
+            d.out_el = ostride;
+            const size_t cb = cpx_bytes(P.type);
+            const size_t ispan = cb * (size_t)((howmany - 1) * idist + (n - 1) * istride + 1);
+            const size_t ospan = cb * (size_t)((howmany - 1) * odist + (n - 1) * ostride + 1);
+            if (!overlap(in, ispan, out, ospan))
+                return launch_dft_stage(P.type, d, s);
+            // in place: the workgroups of a transform read all of it but write parts of it,
+            // so the stage writes a copy of the output span (holding its other values too)
+            Scratch mirror;
+            int rc = mirror.get(ospan, s);
+            if (rc)
+                return rc;
+            if (hipMemcpyAsync(mirror.p, out, ospan, hipMemcpyDeviceToDevice, s) != hipSuccess)
+                return fail(-3, "device copy failed");
+            d.out = mirror.p;
+            if ((rc = launch_dft_stage(P.type, d, s)))
+                return rc;
+            return hipMemcpyAsync(out, mirror.p, ospan, hipMemcpyDeviceToDevice, s) == hipSuccess
+                       ? 0 : fail(-3, "device copy failed");
+        }
+    }
+    Scratch tmp;
+    if (int rc = tmp.get(cpx_bytes(P.type) * (size_t)n * (size_t)howmany, s))
+        return rc;
+    int rc;
+    if (P.bigp) {
+        d.out = tmp.p;
+        d.out_hi = n;
+        d.out_el = 1;
+        rc = launch_dft_stage(P.type, d, s);
+    } else {
+        LineLaunch a{};
+        a.in = in;
+        a.out = tmp.p;
+        a.tw = P.d_tw;
+        a.pl = &P.lp1;
+        a.io.nlines = howmany * P.Q;
+        a.io.nlo = P.Q;
+        a.io.in_hi = idist;
+        a.io.in_lo = istride;
+        a.io.in_el = (long long)P.Q * istride;
+        a.io.out_hi = n;
+        a.io.out_lo = 1;
+        a.io.out_el = P.Q;
+        a.colmode = 1;
+        rc = launch_lines(P.type, a, s);
+    }
     if (rc == 0) {
         LineLaunch b{};
-        b.in = tmp;
+        b.in = tmp.p;
         b.out = out;
         b.tw = P.d_tw;
         b.pl = &P.lp2;
@@ -1365,7 +1610,6 @@ int run_c2c(DevPlan &P, const void *in, long long istride, long long idist, void
         b.colmode = 1;
         rc = launch_lines(P.type, b, s);
     }
-    (void)hipFreeAsync(tmp, s);
     return rc;
 }
 
@@ -1388,16 +1632,24 @@ static int launch_real_pass(int merge, const void *src, void *dst, const void *s
 }
 
 // Float real rows of half length nc = N1*N2 in two HBM passes, the split / merge fused
-// into the four-step passes (kfa_tile_real.hpp).  Forward: the column pass of z (length
-// N2, four-step twiddle) into tmp, then row pairs of tmp (length N1) split into the bins.
-// Inverse: column pairs of the bins merged and transformed (length N2, twiddle) into tmp,
-// then the rows of tmp (length N1) written as columns of z.  tmp holds (nc + 16) * howmany
-// + 16 values.  Returns 1 (nothing done) when no registered split of nc fits.
-static int run_real_fourstep(DevPlan &H, const RealTables &R, int inverse, const void *in, long long in_hi,
-                             void *out, long long out_hi, long long howmany, void *tmp, hipStream_t s)
+// into the four-step passes (kfa_tile_real.hpp), decided once per plan and direction:
+// the fused tile (length f) and the plain tile of the other half, with their tables.
+// H.rfused[inverse] stays null when no registered split of nc fits.
+static int real_fused_plan(DevPlan &H, int inverse, hipStream_t s, std::shared_ptr<const RealFusedPlan> &out)
 {
-    if (H.type != KFA_F32 || !env_flag("KISSFFT_AMD_FAST", true) || !env_flag("KISSFFT_AMD_REAL_FUSED", true))
-        return 1;
+    std::lock_guard<std::mutex> g(H.mu);
+    if (H.rfused_decided[inverse]) {
+        out = H.rfused[inverse];
+        return 0;
+    }
+    out = nullptr;
+    if (H.type != KFA_F32 || H.single || !env_flag("KISSFFT_AMD_FAST", true) ||
+        !env_flag("KISSFFT_AMD_REAL_FUSED", true)) {
+        H.rfused_decided[inverse] = true;
+        return 0;
+    }
+    if (thread_noalloc())
+        return noalloc_refuse("the fused real four-step plan");
     const long long nc = H.n;
     // the fused mapping sets the length of its own half: N1 (split rows) or N2 (merge columns)
     const int fused_mode = inverse ? TILE_RMERGE : TILE_RSPLIT;
@@ -1418,23 +1670,45 @@ static int run_real_fourstep(DevPlan &H, const RealTables &R, int inverse, const
         if (bestf == 0 || score(v.n) < score(bestf))
             bestf = v.n;
     }
-    if (bestf == 0)
-        return 1;
-    const long long other = nc / bestf;
-    const long long N1 = inverse ? other : bestf, N2 = inverse ? bestf : other;
-    const TileChoice *cf = nullptr, *cp = nullptr;
-    int rc;
-    if ((rc = tile_choice((int)bestf, inverse, fused_mode, s, &cf)) ||
-        (rc = tile_choice((int)other, inverse, plain_mode, s, &cp)))
-        return rc;
-    if (!cf->v || !cp->v || (other / 2) % (cf->v->TL / 2))
-        return 1;
+    if (bestf) {
+        auto rp = std::make_shared<RealFusedPlan>();
+        rp->f = bestf;
+        rp->other = nc / bestf;
+        BufPair fs, rs;
+        int rc;
+        if ((rc = tile_choice((int)bestf, inverse, fused_mode, s, rp->cf)) ||
+            (rc = tile_choice((int)rp->other, inverse, plain_mode, s, rp->cp)) ||
+            (rc = fourstep_twiddles(nc, inverse, fs)) || (rc = real_split_twiddles(nc, (int)bestf, inverse, rs)))
+            return rc;
+        rp->tw_hi = fs.first;
+        rp->tw_lo = fs.second;
+        rp->rtw_a = rs.first;
+        rp->rtw_b = rs.second;
+        if (rp->cf->v && rp->cp->v && (rp->other / 2) % (rp->cf->v->TL / 2) == 0)
+            H.rfused[inverse] = rp;
+    }
+    H.rfused_decided[inverse] = true;
+    out = H.rfused[inverse];
+    return 0;
+}
+
+// Forward: the column pass of z (length N2, four-step twiddle) into tmp, then row pairs of
+// tmp (length N1) split into the bins.  Inverse: column pairs of the bins merged and
+// transformed (length N2, twiddle) into tmp, then the rows of tmp (length N1) written as
+// columns of z.  tmp holds (nc + 16) * howmany + 16 values.
+static int run_real_fourstep(const RealFusedPlan &fp, long long nc, int inverse, const void *in, long long in_hi,
+                             void *out, long long out_hi, long long howmany, void *tmp, hipStream_t s)
+{
+    const long long N1 = inverse ? fp.other : fp.f, N2 = inverse ? fp.f : fp.other;
+    const TileChoice &cf = *fp.cf, &cp = *fp.cp;
     RealTileArgs r{};
     r.nc = nc;
+    r.rtw_a = fp.rtw_a.get();
+    r.rtw_b = fp.rtw_b.get();
     TileArgs a{};
-    if ((rc = fourstep_twiddles(nc, inverse, &a.tw_hi, &a.tw_lo)) ||
-        (rc = real_split_twiddles(nc, (int)bestf, inverse, &r.rtw_a, &r.rtw_b)))
-        return rc;
+    a.tw_hi = fp.tw_hi.get();
+    a.tw_lo = fp.tw_lo.get();
+    int rc;
     if (!inverse) {
         // pass 1: line (b, n1): element n2 at in[b*in_hi + n1 + N1*n2] -> tmp[b*nc + n1 + N1*k2]
         a.in = in;
@@ -1447,7 +1721,7 @@ static int run_real_fourstep(DevPlan &H, const RealTables &R, int inverse, const
         a.out_hi = nc;
         a.out_lo = 1;
         a.out_el = N1;
-        rc = launch_tile(*cp, a, s);
+        rc = launch_tile(cp, a, s);
         // pass 2: rows k2 and N2-k2 of tmp -> bins
         r.in = tmp;
         r.out = out;
@@ -1455,7 +1729,7 @@ static int run_real_fourstep(DevPlan &H, const RealTables &R, int inverse, const
         r.out_hi = out_hi;
         r.other = (int)N2;
         if (rc == 0)
-            rc = launch_rtile(*cf, r, howmany, s);
+            rc = launch_rtile(cf, r, howmany, s);
     } else {
         // pass 1: column pairs (n1, N1-n1) of the bins -> tmp[b*tp + n1 + N1*k2].  Each
         // transform of tmp sits at the same offset in its 128-byte line as its bins, so
@@ -1470,7 +1744,7 @@ static int run_real_fourstep(DevPlan &H, const RealTables &R, int inverse, const
         r.in_hi = in_hi;
         r.out_hi = tp;
         r.other = (int)N1;
-        rc = launch_rtile(*cf, r, howmany, s);
+        rc = launch_rtile(cf, r, howmany, s);
         // pass 2: line (b, k2): element n1 at tmp[b*tp + n1 + N1*k2] -> z[b*out_hi + k2 + N2*k1]
         a.in = tmp_al;
         a.out = out;
@@ -1483,21 +1757,37 @@ static int run_real_fourstep(DevPlan &H, const RealTables &R, int inverse, const
         a.out_lo = 1;
         a.out_el = N2;
         if (rc == 0)
-            rc = launch_tile(*cp, a, s);
+            rc = launch_tile(cp, a, s);
     }
+    return rc;
+}
+
+int prepare_multipass(DevPlan &P, int fk, hipStream_t s)
+{
+    if (P.single)
+        return 0;
+    std::shared_ptr<const BaileyPlan> bp;
+    std::shared_ptr<const RealFusedPlan> fp;
+    int rc = bailey_plan(P, s, bp);
+    if (rc == 0 && fk != FK_C2C)
+        rc = real_fused_plan(P, fk == FK_C2R, s, fp);
     return rc;
 }
 
 // Real rows whose half length needs the multi-pass complex transform.  Float lengths with
 // a registered four-step split take the two fused passes of run_real_fourstep; otherwise
-// the complex pass runs into a stream-ordered temporary (nc values per line) and the
-// split (forward) or merge (inverse) runs as its own HBM pass on that temporary.
+// the complex pass runs into a temporary (nc values per line) and the split (forward) or
+// merge (inverse) runs as its own HBM pass on that temporary.
 static int run_real_multipass(DevPlan &H, const RealTables &R, int inverse, const void *in, long long in_hi,
                               void *out, long long out_hi, long long howmany, hipStream_t s)
 {
     const int nc = H.n;
     const size_t cb = cpx_bytes(H.type);
-    const long long chunk = std::max(1LL, (long long)(kScratchChunk / (cb * (size_t)nc)));
+    const size_t per = multipass_bytes_per_transform(H, inverse ? FK_C2R : FK_R2C) + 1024;
+    const size_t room = scratch_room();
+    if (room < per)
+        return fail(-2, "real nfft=%d needs %zu bytes of scratch per transform; %zu available", 2 * nc, per, room);
+    const long long chunk = std::max(1LL, (long long)(room / per));
     if (howmany > chunk) {  // bounded temporaries, as for multi-pass complex lines
         for (long long b = 0; b < howmany; b += chunk) {
             const int rc = run_real_multipass(H, R, inverse, (const char *)in + (size_t)(b * in_hi) * cb, in_hi,
@@ -1508,8 +1798,15 @@ static int run_real_multipass(DevPlan &H, const RealTables &R, int inverse, cons
         }
         return 0;
     }
-    void *tmp = nullptr;  // room for the fused inverse's aligned layout (16 more values per transform)
-    KFA_HIP(scratch_alloc(&tmp, cb * ((size_t)(nc + 16) * (size_t)howmany + 16), s));
+    std::shared_ptr<const RealFusedPlan> fp;
+    int rc = real_fused_plan(H, inverse, s, fp);
+    if (rc)
+        return rc;
+    Scratch tmp;  // room for the fused inverse's aligned layout (16 more values per transform)
+    if ((rc = tmp.get(cb * ((size_t)(nc + 16) * (size_t)howmany + 16), s)))
+        return rc;
+    if (fp)
+        return run_real_fourstep(*fp, nc, inverse, in, in_hi, out, out_hi, howmany, tmp.p, s);
     auto pass = [&](int merge, const void *src, void *dst, long long hi) {
         switch (H.type) {
         case KFA_F32: return launch_real_pass<F32>(merge, src, dst, R.d_stw, nc, hi, howmany, s);
@@ -1517,21 +1814,15 @@ static int run_real_multipass(DevPlan &H, const RealTables &R, int inverse, cons
         default: return launch_real_pass<Q31>(merge, src, dst, R.d_stw, nc, hi, howmany, s);
         }
     };
-    int rc = run_real_fourstep(H, R, inverse, in, in_hi, out, out_hi, howmany, tmp, s);
-    if (rc != 1) {
-        (void)hipFreeAsync(tmp, s);
-        return rc;
-    }
     if (!inverse) {
-        rc = run_c2c(H, in, 1, in_hi, tmp, 1, nc, howmany, s);
+        rc = run_c2c(H, in, 1, in_hi, tmp.p, 1, nc, howmany, s);
         if (rc == 0)
-            rc = pass(0, tmp, out, out_hi);
+            rc = pass(0, tmp.p, out, out_hi);
     } else {
-        rc = pass(1, in, tmp, in_hi);
+        rc = pass(1, in, tmp.p, in_hi);
         if (rc == 0)
-            rc = run_c2c(H, tmp, 1, nc, out, 1, out_hi, howmany, s);
+            rc = run_c2c(H, tmp.p, 1, nc, out, 1, out_hi, howmany, s);
     }
-    (void)hipFreeAsync(tmp, s);
     return rc;
 }
 
@@ -1636,14 +1927,10 @@ void kfa_get_stats(kfa_stats *st)
     st->d2h_bytes = g_d2h.load();
 }
 
-int kfa_c2c(const kfa_cplan *pl, const void *in, long long istride, long long idist, void *out,
-            long long odist, long long howmany, void *stream, int sync)
+static int c2c_call(const kfa_cplan *pl, const void *in, long long istride, long long idist, void *out,
+                    long long odist, long long howmany, hipStream_t s, int dev, int sync)
 {
-    hipStream_t s;
-    int dev;
-    int rc = begin(&s, stream, &dev);
-    if (rc)
-        return rc;
+    int rc;
     std::shared_ptr<DevPlan> P;
     if ((rc = build_cplan(pl, dev, P)))
         return rc;
@@ -1656,18 +1943,26 @@ int kfa_c2c(const kfa_cplan *pl, const void *in, long long istride, long long id
     Operand I, O;
     if ((rc = stage_in(in, in_bytes, s, I)) == 0 &&
         (rc = stage_out(out, out_bytes, s, O, howmany > 1 && odist > n)) == 0) {
-        const bool same = I.dev == O.dev && !I.staged && !O.staged;
-        const bool inplace_ok = !same || (P->single && istride == 1 && idist == odist);
+        // in place is safe when every transform reads and writes its own span, except for
+        // the one-pass global DFT stage (its workgroups read a whole transform but write
+        // parts of it)
+        const bool aliased = !I.staged && !O.staged && overlap(I.dev, in_bytes, O.dev, out_bytes);
+        const bool inplace_ok = !aliased || (I.dev == O.dev && istride == 1 && idist == odist &&
+                                             !(P->bigp && P->Q == 1));
         if (inplace_ok) {
             rc = run_c2c(*P, I.dev, istride, idist, O.dev, 1, odist, howmany, s);
         } else {
-            // device in == out with a layout the single pass cannot overwrite in place
-            void *tmp = nullptr;
-            KFA_HIP(scratch_alloc(&tmp, out_bytes, s));
-            rc = run_c2c(*P, I.dev, istride, idist, tmp, 1, odist, howmany, s);
+            // overlapping device buffers with layouts that cannot be overwritten in place
+            Scratch tmp;
+            rc = tmp.get(out_bytes, s);
+            // values between the rows (odist > n) keep the caller's bytes
+            if (rc == 0 && howmany > 1 && odist > n &&
+                hipMemcpyAsync(tmp.p, O.dev, out_bytes, hipMemcpyDeviceToDevice, s) != hipSuccess)
+                rc = fail(-3, "device copy failed");
             if (rc == 0)
-                KFA_HIP(hipMemcpyAsync(O.dev, tmp, out_bytes, hipMemcpyDeviceToDevice, s));
-            (void)hipFreeAsync(tmp, s);
+                rc = run_c2c(*P, I.dev, istride, idist, tmp.p, 1, odist, howmany, s);
+            if (rc == 0 && hipMemcpyAsync(O.dev, tmp.p, out_bytes, hipMemcpyDeviceToDevice, s) != hipSuccess)
+                rc = fail(-3, "device copy failed");
         }
         if (rc == 0)
             rc = finish_out(O, s);
@@ -1675,6 +1970,17 @@ int kfa_c2c(const kfa_cplan *pl, const void *in, long long istride, long long id
     release(I, s);
     release(O, s);
     return end_call(s, sync || I.staged || O.staged, rc);
+}
+
+int kfa_c2c(const kfa_cplan *pl, const void *in, long long istride, long long idist, void *out,
+            long long odist, long long howmany, void *stream, int sync)
+{
+    hipStream_t s;
+    int dev;
+    int rc = begin(&s, stream, &dev);
+    if (rc)
+        return rc;
+    return c2c_call(pl, in, istride, idist, out, odist, howmany, s, dev, sync);
 }
 
 static int real_common(const kfa_rplan *pl, const void *in, long long idist, void *out, long long odist,
@@ -1707,14 +2013,32 @@ static int real_common(const kfa_rplan *pl, const void *in, long long idist, voi
         rc = stage_out(out, inverse ? real_bytes_out : cpx_bytes_out, s, O,
                        howmany > 1 && odist > (inverse ? 2 * nc : nc + 1));
     if (rc == 0) {
-        if (!inverse && (idist & 1))
+        const size_t in_b = inverse ? cpx_bytes_in : real_bytes_in, out_b = inverse ? real_bytes_out : cpx_bytes_out;
+        // rows of different length on the two sides: overlapping buffers are only safe for
+        // one single-pass transform in place (its line is read whole before it is written)
+        const bool aliased = !I.staged && !O.staged && overlap(I.dev, in_b, O.dev, out_b) &&
+                             !(howmany == 1 && I.dev == O.dev && P->single);
+        Scratch tmp;
+        void *dst = O.dev;
+        if (aliased) {
+            rc = tmp.get(out_b, s);
+            if (rc == 0 && howmany > 1 && odist > (inverse ? 2 * nc : nc + 1) &&
+                hipMemcpyAsync(tmp.p, O.dev, out_b, hipMemcpyDeviceToDevice, s) != hipSuccess)
+                rc = fail(-3, "device copy failed");
+            dst = tmp.p;
+        }
+        if (rc)
+            ;
+        else if (!inverse && (idist & 1))
             rc = fail(-2, "real input distance must be even (complex-pair loads)");
         else if (inverse && (odist & 1))
             rc = fail(-2, "real output distance must be even (complex-pair stores)");
         else if (!inverse)
-            rc = run_real_rows(*P, *R, 0, I.dev, idist / 2, O.dev, odist, howmany, s);
+            rc = run_real_rows(*P, *R, 0, I.dev, idist / 2, dst, odist, howmany, s);
         else
-            rc = run_real_rows(*P, *R, 1, I.dev, idist, O.dev, odist / 2, howmany, s);
+            rc = run_real_rows(*P, *R, 1, I.dev, idist, dst, odist / 2, howmany, s);
+        if (rc == 0 && aliased && hipMemcpyAsync(O.dev, tmp.p, out_b, hipMemcpyDeviceToDevice, s) != hipSuccess)
+            rc = fail(-3, "device copy failed");
         if (rc == 0)
             rc = finish_out(O, s);
     }
@@ -1742,11 +2066,20 @@ static int run_axis(DevPlan &P, const void *in, void *out, long long outer, long
     const long long len = P.n;
     if (inner == 1)  // the last axis: contiguous lines
         return run_c2c(P, in, 1, len, out, 1, len, outer, s);
-    if (!P.single)
-        return fail(-2, "N-D axis of %lld points with inner stride %lld: not yet supported", len, inner);
+    if (!P.single) {
+        // an axis longer than one LDS line: the strided lines of each outer slice through the
+        // multi-pass path (element e of line j at j + e*inner)
+        const size_t cb = cpx_bytes(P.type);
+        for (long long o = 0; o < outer; ++o) {
+            const size_t off = cb * (size_t)(o * len * inner);
+            if (int rc = run_c2c(P, (const char *)in + off, inner, 1, (char *)out + off, inner, 1, inner, s))
+                return rc;
+        }
+        return 0;
+    }
     if (P.type == KFA_F32) {
-        const TileChoice *tc = nullptr;
-        int rc = tile_choice((int)len, P.inverse, TILE_COLS, s, &tc);
+        std::shared_ptr<const TileChoice> tc;
+        int rc = tile_choice((int)len, P.inverse, TILE_COLS, s, tc);
         if (rc)
             return rc;
         if (tc->v) {
@@ -1801,8 +2134,8 @@ int run_nd(const std::vector<std::shared_ptr<DevPlan>> &P, const int *dims, cons
     int last = ndims;  // axes [0, last) run one pass each
     if (ndims >= 2 && P[ndims - 1]->type == KFA_F32) {
         // the last two axes in one pass when a plane of them fits in LDS
-        const PlaneChoice *pc = nullptr;
-        if ((rc = plane_choice(dims[ndims - 2], dims[ndims - 1], P[ndims - 1]->inverse, s, &pc)))
+        std::shared_ptr<const PlaneChoice> pc;
+        if ((rc = plane_choice(dims[ndims - 2], dims[ndims - 1], P[ndims - 1]->inverse, s, pc)))
             return rc;
         if (pc->v) {
             PlaneArgs pa{in, out, pc->stw_a, pc->stw_b, howmany * (total / ((long long)dims[ndims - 2] * dims[ndims - 1]))};
@@ -1845,22 +2178,22 @@ int run_ndr(DevPlan &H, const RealTables &R, const std::vector<std::shared_ptr<D
         return rc;
     }
     // the input is left untouched: the axes run out of place into a temporary
-    void *tmp = nullptr;
+    Scratch tmp;
     const size_t tb = cpx_bytes(H.type) * (size_t)(rows * nbins * howmany);
-    KFA_HIP(scratch_alloc(&tmp, tb, s));
+    if ((rc = tmp.get(tb, s)))
+        return rc;
     long long outer = howmany, inner = rows * nbins;
     const void *src = in;
     for (int k = 0; k < nlead && rc == 0; ++k) {
         inner /= dims[k];
-        rc = run_axis(*P[k], src, tmp, outer, inner, s);
+        rc = run_axis(*P[k], src, tmp.p, outer, inner, s);
         outer *= dims[k];
-        src = tmp;
+        src = tmp.p;
     }
     if (rc == 0 && nlead == 0)
-        rc = hipMemcpyAsync(tmp, in, tb, hipMemcpyDeviceToDevice, s) == hipSuccess ? 0 : fail(-3, "copy failed");
+        rc = hipMemcpyAsync(tmp.p, in, tb, hipMemcpyDeviceToDevice, s) == hipSuccess ? 0 : fail(-3, "copy failed");
     if (rc == 0)
-        rc = run_real_rows(H, R, 1, tmp, nbins, out, nreal / 2, rows * howmany, s);
-    (void)hipFreeAsync(tmp, s);
+        rc = run_real_rows(H, R, 1, tmp.p, nbins, out, nreal / 2, rows * howmany, s);
     return rc;
 }
 
@@ -1891,7 +2224,15 @@ int kfa_nd(int ndims, const int *dims, const kfa_cplan *axes, const void *in, vo
     if (rc == 0)
         rc = stage_out(out, bytes, s, O);
     if (rc == 0) {
-        rc = run_nd(P, dims, I.dev, O.dev, howmany, s);
+        // overlapping but not identical buffers: the passes run into a temporary
+        const bool aliased = !I.staged && !O.staged && I.dev != O.dev && overlap(I.dev, bytes, O.dev, bytes);
+        Scratch tmp;
+        if (aliased)
+            rc = tmp.get(bytes, s);
+        if (rc == 0)
+            rc = run_nd(P, dims, I.dev, aliased ? tmp.p : O.dev, howmany, s);
+        if (rc == 0 && aliased && hipMemcpyAsync(O.dev, tmp.p, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess)
+            rc = fail(-3, "device copy failed");
         if (rc == 0)
             rc = finish_out(O, s);
     }
@@ -1931,7 +2272,18 @@ int kfa_ndr(int ndims, const int *dims, const kfa_rplan *rpl, const kfa_cplan *a
     if (rc == 0)
         rc = stage_out(out, inverse ? real_bytes : cpx_bytes_, s, O);
     if (rc == 0) {
-        rc = run_ndr(*H, *R, P, dims, inverse, I.dev, O.dev, howmany, s);
+        // the forward real rows write rows of nbins complex over the following rows' real
+        // input: overlapping buffers run into a temporary (the inverse reads its input
+        // whole into its own temporary first)
+        const size_t in_b = inverse ? cpx_bytes_ : real_bytes, out_b = inverse ? real_bytes : cpx_bytes_;
+        const bool aliased = !inverse && !I.staged && !O.staged && overlap(I.dev, in_b, O.dev, out_b);
+        Scratch tmp;
+        if (aliased)
+            rc = tmp.get(out_b, s);
+        if (rc == 0)
+            rc = run_ndr(*H, *R, P, dims, inverse, I.dev, aliased ? tmp.p : O.dev, howmany, s);
+        if (rc == 0 && aliased && hipMemcpyAsync(O.dev, tmp.p, out_b, hipMemcpyDeviceToDevice, s) != hipSuccess)
+            rc = fail(-3, "device copy failed");
         if (rc == 0)
             rc = finish_out(O, s);
     }
@@ -1951,52 +2303,30 @@ int kfa_stream_sync(void *stream)
     return 0;
 }
 
+// Drops the engine's caches.  Plans and tables still held elsewhere -- calls in flight on
+// other threads, requests queued in a server, multi-batches -- stay valid until their last
+// holder lets go; everything released is freed here (hipFree waits for the device), or at
+// a later cleanup for what is released later.  (Graphs that a caller captured from *_async
+// calls hold no reference: they must not be replayed after a cleanup.)
 void kfa_cleanup(void)
 {
-    auto &C = cache();
-    std::lock_guard<std::mutex> g(C.mu);
-    for (auto &kv : C.cplans)
-        if (kv.second->d_tw) {
-            (void)hipFree(kv.second->d_tw);
-            kv.second->d_tw = nullptr;
-        }
-    for (auto &kv : C.rtabs)
-        if (kv.second->d_stw) {
-            (void)hipFree(kv.second->d_stw);
-            kv.second->d_stw = nullptr;
-        }
-    for (auto &kv : C.cplans)
-        for (FastChoice &fc : kv.second->fast)
-            if (fc.d_stw) {
-                (void)hipFree(fc.d_stw);
-                fc.d_stw = nullptr;
-            }
-    C.cplans.clear();
-    C.rtabs.clear();
-    scratch_trim();
-    auto &T = tile_cache();
-    std::lock_guard<std::mutex> gt(T.mu);
-    for (auto &kv : T.choice)
-        if (kv.second.d_stw)
-            (void)hipFree(kv.second.d_stw);
-    auto free_pairs = [](auto &m) {
-        for (auto &kv : m) {
-            (void)hipFree(kv.second.first);
-            (void)hipFree(kv.second.second);
-        }
-    };
-    free_pairs(T.fs);
-    free_pairs(T.rs);
-    for (auto &kv : T.plane)
-        for (void *p : {kv.second.stw_a, kv.second.stw_b})
-            if (p)
-                (void)hipFree(p);
-    T.plane.clear();
-    T.choice.clear();
-    T.fs.clear();
-    T.rs.clear();
-    blue_release();
+    {
+        auto &C = cache();
+        std::lock_guard<std::mutex> g(C.mu);
+        C.cplans.clear();
+        C.rtabs.clear();
+    }
+    {
+        auto &T = tile_cache();
+        std::lock_guard<std::mutex> gt(T.mu);
+        T.plane.clear();
+        T.choice.clear();
+        T.fs.clear();
+        T.rs.clear();
+    }
     conv_release();
+    free_deferred();
+    scratch_trim();
 }
 
 }  // extern "C"

@@ -15,27 +15,73 @@
 
 namespace kfa {
 
+// Device memory with shared ownership: the plan cache, calls in flight, queued server
+// requests and multi-batches each hold a reference.  When the last holder lets go the
+// pointer is queued, and free_deferred() -- called by kfa_cleanup -- frees the queue
+// (hipFree waits for the device first).  So a cleanup never frees memory that a queued or
+// running kernel still reads, and no HIP call is made from a destructor, which may run
+// inside a stream callback.
+using DevBuf = std::shared_ptr<void>;
+int dev_upload(const void *host, size_t bytes, DevBuf &out);
+int dev_alloc(size_t bytes, DevBuf &out);
+void free_deferred();
+
 // The fast variant chosen for one transform kind of a plan, with its stage table.
 struct FastChoice {
     bool decided = false;
     const FastVariant *v = nullptr;
+    DevBuf stw_mem;
     void *d_stw = nullptr;
 };
+
+// A float tile variant (kfa_tile.hpp) with its stage table.
+struct TileChoice {
+    const TileVariant *v = nullptr;
+    DevBuf stw_mem;
+    void *d_stw = nullptr;
+};
+
+// Float Bailey four-step split of one multi-pass line: pass 1 the N1 columns of length N2
+// (with the four-step twiddle), pass 2 the N2 rows of length N1.
+struct BaileyPlan {
+    long long N1 = 0, N2 = 0;
+    std::shared_ptr<const TileChoice> c1, c2;
+    DevBuf tw_hi, tw_lo;
+};
+
+// Float real lines whose half is multi-pass, the split / merge fused into the four-step
+// passes: the fused tile (length f) and the plain tile (length other = nc / f).
+struct RealFusedPlan {
+    long long f = 0, other = 0;
+    std::shared_ptr<const TileChoice> cf, cp;
+    DevBuf tw_hi, tw_lo, rtw_a, rtw_b;
+};
+
+struct BlueChoice;
 
 struct DevPlan {
     int type = 0, n = 0, inverse = 0;
     std::vector<int> radix;  // execution order (innermost factor first)
     std::vector<unsigned char> h_tw;  // host copy of the reference twiddle table
+    DevBuf tw_mem;
     void *d_tw = nullptr;    // n complex
     bool single = true;
     LinePlan lp{};           // single pass
     int L1 = 0, Q = 0;       // four-step: L1 * Q == n
+    bool bigp = false;       // first stage radix L1 above one LDS line: a global-memory DFT pass
     LinePlan lp1{}, lp2{};
-    std::mutex mu;           // guards `fast`
+    std::mutex mu;           // guards the choices below (each decided once)
     FastChoice fast[3];      // per FK_C2C / FK_R2C / FK_C2R
+    bool bailey_decided = false;
+    std::shared_ptr<const BaileyPlan> bailey;
+    bool rfused_decided[2] = {false, false};
+    std::shared_ptr<const RealFusedPlan> rfused[2];  // per direction
+    bool blue_decided = false;
+    std::shared_ptr<const BlueChoice> blue;
 };
 
 struct RealTables {
+    DevBuf stw_mem;
     void *d_stw = nullptr;  // nfft/4 complex
 };
 
@@ -46,8 +92,43 @@ int line_cap(int type);
 int current_device(int *dev);
 // stream-ordered scratch from the engine's per-device pool (free with hipFreeAsync)
 hipError_t scratch_alloc(void **p, size_t bytes, hipStream_t s);
-// multi-pass transforms process batches in chunks whose temporary fits this many bytes
+
+// Scratch of one call: carved from the calling thread's current workspace when one is set
+// (a multi-batch's, allocated at its creation; the no-allocation entry point's static
+// device array), else stream-ordered from the device pool.  Released when the object
+// goes out of scope (workspace carving is a stack: releases come in reverse order).
+struct Workspace {
+    char *base = nullptr;
+    size_t size = 0, used = 0;
+};
+Workspace *&thread_workspace();
+// set while a no-allocation entry point runs: every allocation site fails instead
+bool &thread_noalloc();
+int noalloc_refuse(const char *what);
+struct Scratch {
+    void *p = nullptr;
+    Scratch() = default;
+    Scratch(const Scratch &) = delete;
+    Scratch &operator=(const Scratch &) = delete;
+    int get(size_t bytes, hipStream_t s);
+    ~Scratch();
+
+  private:
+    hipStream_t s_ = nullptr;
+    Workspace *ws_ = nullptr;
+    size_t mark_ = 0;
+    bool pooled_ = false;
+};
+// Multi-pass batches run in chunks whose temporaries fit this many bytes: the free part
+// of the thread's workspace, else the chunk limit (KISSFFT_AMD_CHUNK_MB, default below).
 constexpr size_t kScratchChunk = (size_t)1 << 30;
+size_t scratch_room();
+// scratch bytes one transform of plan P needs on the multi-pass paths (complex: kind
+// FK_C2C; real: FK_R2C / FK_C2R with P the half-length plan), 0 for single-pass plans
+size_t multipass_bytes_per_transform(DevPlan &P, int fk);
+// decides every lazily built choice a multi-pass transform of P will use (tile variants,
+// four-step tables), so that later calls allocate nothing but their scratch
+int prepare_multipass(DevPlan &P, int fk, hipStream_t s);
 bool device_accessible(const void *p);
 
 // device plans (cached; thread-safe)
@@ -71,11 +152,22 @@ int run_ndr(DevPlan &H, const RealTables &R, const std::vector<std::shared_ptr<D
 
 // A caller buffer: used in place when the device can access it, else staged through a
 // stream-ordered device copy (stage_in copies host -> device; finish_out device -> host).
+// The staging copy is returned to the pool by release() or, on an early return, by the
+// destructor.
 struct Operand {
     const void *host = nullptr;  // non-null when staged
     void *dev = nullptr;
     size_t bytes = 0;
     bool staged = false;
+    hipStream_t s = nullptr;     // the stream the staging copy is ordered on
+    Operand() = default;
+    Operand(const Operand &) = delete;
+    Operand &operator=(const Operand &) = delete;
+    ~Operand()
+    {
+        if (staged && dev)
+            (void)hipFreeAsync(dev, s);
+    }
 };
 int stage_in(const void *p, size_t bytes, hipStream_t s, Operand &op);
 // gaps: the output span holds values the call must not touch (rows further apart than
@@ -89,13 +181,26 @@ bool env_flag(const char *name, bool dflt);
 std::vector<unsigned char> float_twiddles(long long n, int inverse, long long count, long long step);
 // per-stage twiddle table of schedule `rad` (the fast kernels' layout) from a full table
 int make_stage_table_raw(const unsigned char *h_tw, int type, long long n, const std::vector<int> &rad, int pad,
-                         int fmt, void **d_out);
+                         int fmt, DevBuf &out);
 
-// whether a float plan runs Bluestein's kernel (its tables built on return)
-int blue_prepare(const DevPlan &P, bool *applies);
+// whether a float plan runs Bluestein's kernel (its tables built on return; the caller may
+// hold them in `keep`)
+int blue_prepare(DevPlan &P, bool *applies, std::shared_ptr<const BlueChoice> *keep = nullptr);
 // one Bluestein launch for plan P; b carries buffers, segments, counts and distances
 struct BlueArgs;
-int blue_launch(const DevPlan &P, BlueArgs &b, hipStream_t s);
+int blue_launch(DevPlan &P, BlueArgs &b, hipStream_t s);
+
+// Tuning (first-use timing of variants) runs on a private stream of the device, never on
+// a caller's stream, and not at all while the caller's stream is being captured.
+int tune_stream(hipStream_t *s);
+bool stream_capturing(hipStream_t s);
+// Plan tables are uploaded with synchronous copies; while this object lives the calling
+// thread's capture mode is relaxed, so such uploads do not invalidate a capture in progress.
+struct RelaxCapture {
+    hipStreamCaptureMode mode = hipStreamCaptureModeRelaxed;
+    RelaxCapture() { (void)hipThreadExchangeStreamCaptureMode(&mode); }
+    ~RelaxCapture() { (void)hipThreadExchangeStreamCaptureMode(&mode); }
+};
 
 // releases the fast-convolution device plans (kfa_conv.hip)
 void conv_release();

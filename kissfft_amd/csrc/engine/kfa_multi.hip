@@ -12,8 +12,10 @@
 //     kernel) joins one persistent launch of k_multi (kfa_lines.hpp): a grid sized to the
 //     device walks a global list of tiles with the general LDS-resident line code, any mix of
 //     radices and kinds together, bit-exact in the fixed-point builds;
-//   * items longer than one LDS line run as their own multi-pass launches (these allocate
-//     scratch, so a multi-batch holding them cannot be captured).
+//   * items longer than one LDS line run as their own multi-pass launches; their temporaries
+//     are carved from one workspace allocated at creation (sized for the largest item's
+//     chunk), and their lazily built choices (tile variants, four-step tables) are decided
+//     at creation too, so these launches are capturable as well.
 // KISSFFT_AMD_MULTI_PERSISTENT=1 puts every single-pass item into the persistent launch.
 // Execution issues the launches back to back on the caller's stream (largest group first);
 // capture records them into one hipGraph.  KISSFFT_AMD_MULTI_HELPERS=n (at most 8) forks
@@ -57,7 +59,9 @@ struct Multi {
         long long howmany;
         int n;
     };
-    std::vector<Long> longs;             // multi-pass (allocate scratch)
+    std::vector<Long> longs;             // multi-pass (their temporaries in `ws`)
+    DevBuf ws_mem;
+    Workspace ws;
     struct Group {                       // one segmented launch of a compiled row kernel
         const FastVariant *v;
         FastArgs a;
@@ -67,6 +71,7 @@ struct Multi {
     int group_items = 0;
     struct BlueGroup {                   // one segmented Bluestein launch (float primes > 16)
         std::shared_ptr<DevPlan> P;
+        std::shared_ptr<const BlueChoice> bc;  // the tables the launches (and a graph) use
         BlueArgs a;
     };
     std::vector<BlueGroup> blue;
@@ -145,8 +150,15 @@ static int run_multi(Multi &M, hipStream_t s)
         default: rc = launch_multi<Q31>(M, sp); break;
         }
     }
-    for (size_t i = 0; i < M.longs.size() && rc == 0; ++i)
-        rc = run_long(M.longs[i], s);
+    if (!M.longs.empty()) {
+        // the long items' temporaries come from the multi-batch's workspace
+        Workspace *prev = thread_workspace();
+        M.ws.used = 0;
+        thread_workspace() = &M.ws;
+        for (size_t i = 0; i < M.longs.size() && rc == 0; ++i)
+            rc = run_long(M.longs[i], s);
+        thread_workspace() = prev;
+    }
     // join (also after a failure: the helper streams must leave a capture)
     for (size_t i = 1; i < st.size(); ++i) {
         check(hipEventRecord(ev[i], st[i]));
@@ -197,6 +209,7 @@ void *kfa_multi_create(int type, const kfa_multi_item *items, int nitems)
     std::vector<std::shared_ptr<DevPlan>> iplan(nitems);
     std::vector<std::shared_ptr<RealTables>> irtab(nitems);
     long long tiles = 0;
+    size_t ws_bytes = 0;
     for (int i = 0; i < nitems; ++i) {
         const kfa_multi_item &it = items[i];
         const bool real = it.kind == KFA_SERVE_R2C || it.kind == KFA_SERVE_C2R;
@@ -224,6 +237,14 @@ void *kfa_multi_create(int type, const kfa_multi_item *items, int nitems)
         irtab[i] = R;
         const long long nc = P->n;
         if (!P->single) {
+            const int fk = it.kind == KFA_SERVE_R2C ? FK_R2C : it.kind == KFA_SERVE_C2R ? FK_C2R : FK_C2C;
+            if ((rc = prepare_multipass(*P, fk, s0)))
+                return nullptr;
+            // workspace: the item's chunk of transforms (as the call without a workspace
+            // would run it), at least one transform
+            const size_t per = multipass_bytes_per_transform(*P, fk) + 1024;
+            const size_t want = std::min((size_t)it.howmany * per, std::max(per, scratch_room()));
+            ws_bytes = std::max(ws_bytes, want + 4096);
             M->longs.push_back(Multi::Long{it.kind, P, R, it.in, it.out, it.howmany, it.nfft});
             continue;
         }
@@ -273,6 +294,12 @@ void *kfa_multi_create(int type, const kfa_multi_item *items, int nitems)
         M->h.push_back(mi);
     }
     M->ntiles = tiles;
+    if (ws_bytes) {
+        if (dev_alloc(ws_bytes, M->ws_mem))
+            return nullptr;
+        M->ws.base = static_cast<char *>(M->ws_mem.get());
+        M->ws.size = ws_bytes;
+    }
     // one segmented launch per (kind, length) and at most KFA_MAX_SEG items
     for (auto &kv : by_key) {
         const std::vector<int> &idx = kv.second;
@@ -311,6 +338,9 @@ void *kfa_multi_create(int type, const kfa_multi_item *items, int nitems)
             const size_t g1 = std::min(idx.size(), g0 + KFA_MAX_SEG);
             Multi::BlueGroup G{};
             G.P = iplan[idx[g0]];
+            bool applies = false;
+            if (blue_prepare(*G.P, &applies, &G.bc) || !applies)
+                return nullptr;
             const long long n = G.P->n;
             G.a.in_hi = G.a.out_hi = n;
             G.a.in_el = G.a.out_el = 1;
@@ -388,8 +418,6 @@ int kfa_multi_capture(void *m, void *stream)
     Multi *M = static_cast<Multi *>(m);
     if (!M || !stream)
         return fail(-2, "multi-batch capture needs a stream");
-    if (!M->longs.empty())
-        return fail(-2, "multi-batch capture: items longer than one LDS line allocate scratch; not capturable");
     hipStream_t s = (hipStream_t)stream;
     if (M->gexec) {
         (void)hipGraphExecDestroy(M->gexec);

@@ -392,13 +392,12 @@ long long kfa_server_submit(void *srv, int kind, const kfa_cplan *cp, const kfa_
     r->stage_in = !device_accessible(in);
     r->stage_out = !device_accessible(out);
     r->need = (r->stage_in ? r->in_bytes : 0) + (r->stage_out ? r->out_bytes : 0);
-    // multi-pass scratch: the four-step temporary (complex: n values per transform; real:
-    // n/2, plus the n/2 of the split/merge temporary), each bounded by the chunking of
-    // multi-pass batches
+    // multi-pass scratch: the temporaries of one chunk of transforms (the chunking of
+    // multi-pass batches bounds them)
     if (!r->P->single) {
-        const bool real = kind == KFA_SERVE_R2C || kind == KFA_SERVE_C2R;
-        const size_t one = cpx_bytes(S->type) * (size_t)r->P->n * (size_t)howmany;
-        r->need += (real ? 2 : 1) * std::min(one, kScratchChunk);
+        const int fk = kind == KFA_SERVE_R2C ? FK_R2C : kind == KFA_SERVE_C2R ? FK_C2R : FK_C2C;
+        const size_t per = multipass_bytes_per_transform(*r->P, fk) + 1024;
+        r->need += std::min((size_t)howmany * per, std::max(per, scratch_room()));
     }
     r->t_submit = now_s();
     {
@@ -472,7 +471,7 @@ long long kfa_server_submit_nd(void *srv, int kind, int ndims, const int *dims, 
     if (kind == KFA_SERVE_C2R)  // the inverse runs its complex axes into a temporary
         r->need += cpx_all;
     if (multipass)
-        r->need += 2 * kScratchChunk;
+        r->need += 2 * scratch_room();
     r->t_submit = now_s();
     {
         std::lock_guard<std::mutex> g(S->mu);

@@ -85,9 +85,18 @@ def test_float_mix(layout):
             np.testing.assert_array_equal(y.cpu().numpy(), x.cpu().numpy())
             continue
         assert rel_l2(got, fp64_ref(k, n, x.cpu().numpy())) <= ABS_FLOAT_BOUND, (k, n)
-    # a long item cannot be captured (it allocates scratch)
-    with pytest.raises(kissfft_amd.KissFFTError):
-        m.capture(torch.cuda.Stream())
+    # long items take their temporaries from the multi-batch's workspace: the batch
+    # captures, and a replay gives the eager results
+    eager = [y.clone() for _, y in bufs]
+    st = torch.cuda.Stream()
+    m.capture(st)
+    with torch.cuda.stream(st):
+        for _, y in bufs:
+            y.fill_(7.0)
+    m.replay(st)
+    st.synchronize()
+    for (k, n, b), (_, y), e in zip(FLOAT_ITEMS, bufs, eager):
+        assert torch.equal(y, e), (k, n, b)
     m.close()
 
 
@@ -207,3 +216,48 @@ def test_helper_streams_same_results(monkeypatch):
             say("helpers replay", rep, k, n)
             assert torch.equal(y, w), (rep, k, n)
     forked.close()
+
+
+def test_graph_capture_with_multipass_items():
+    """Items longer than one LDS line (float 65536 and 2^20 complex, 2^20 real both ways,
+    Q15 65536) take their temporaries from the multi-batch's own workspace, so the batch
+    captures; every replay equals an eager execution, and the results match FP64 / the
+    oracle."""
+    import torch
+    dev = torch.device("cuda:0")
+    for dtype, items in (("float", [(C2C, 65536, 3), (C2C_INV, 1048576, 2), (R2C, 1048576, 2),
+                                    (C2R, 1048576, 2), (C2C, 256, 30), (R2C, 40000, 2)]),
+                         ("int16_t", [(C2C, 65536, 3), (C2C_INV, 32768, 2), (C2C, 512, 20)])):
+        lib = kissfft_amd.load(dtype)
+        bufs = [make(torch, dev, k, n, b, 700 + i, dtype) for i, (k, n, b) in enumerate(items)]
+        m = lib.multi([(k, n, x, y, b) for (k, n, b), (x, y) in zip(items, bufs)])
+        info = m.info()
+        say("multipass capture", dtype, info)
+        assert info["multipass_items"] >= 2
+        m.exec()
+        torch.cuda.synchronize()
+        eager = [y.clone() for _, y in bufs]
+        st = torch.cuda.Stream()
+        m.capture(st)
+        for rep in range(3):
+            with torch.cuda.stream(st):
+                for _, y in bufs:
+                    y.fill_(7)
+            m.replay(st)
+            st.synchronize()
+            for (k, n, b), (_, y), e in zip(items, bufs, eager):
+                say("replay", dtype, rep, "item", k, n, b)
+                assert torch.equal(y, e), (dtype, rep, k, n)
+            torch.cuda.synchronize()
+            lib.fft_batch(lib.fft_alloc(256), bufs[0][0][:1, :256].contiguous(),
+                          torch.empty(1, 256, 2, dtype=bufs[0][0].dtype, device=dev), 1, 256, 256)
+        O = Oracle("int16_t") if dtype == "int16_t" else None
+        for (k, n, b), (x, y) in zip(items, bufs):
+            xs, got = x.cpu().numpy(), y.cpu().numpy()
+            if O is not None:
+                want = O.fft(xs, inverse=(k == C2C_INV))
+                np.testing.assert_array_equal(got, want, err_msg=f"q15 kind {k} n {n}")
+            else:
+                got = got if k == C2R else cplx(got)
+                assert rel_l2(got, fp64_ref(k, n, xs)) <= ABS_FLOAT_BOUND, (k, n)
+        m.close()
