@@ -38,33 +38,65 @@ def alg_bytes(n: int) -> int:
     return 4 * n + 8 * (n // 2 + 1)
 
 
-def cpu_baseline(threads: int) -> dict | None:
-    """Reference build timed on the host: r2c+c2r at both sizes, 8192 signals each."""
-    exe = os.path.join(ROOT, "oracle", "_ref", "cpu_baseline-float")
-    sample = 8192
-    if not os.path.exists(exe):
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _ref_pass(exe: str, kind: str, n: int, sample: int, threads: int, mode: str) -> dict | None:
+    r = subprocess.run([exe, kind, str(n), str(sample), str(threads), "5", mode], capture_output=True,
+                       text=True, timeout=300)
+    if r.returncode != 0:
+        log("cpu baseline failed:", r.stdout, r.stderr)
         return None
-    total_bytes = 0.0
-    total_s = 0.0
-    for n in SIZES:
-        for kind in ("r2c", "c2r"):
-            r = subprocess.run([exe, kind, str(n), str(sample), str(threads), "3"], capture_output=True,
-                               text=True, timeout=300)
-            if r.returncode != 0:
-                log("cpu baseline failed:", r.stdout, r.stderr)
-                return None
-            d = json.loads(r.stdout)
-            total_s += d["median_s"]
-            total_bytes += alg_bytes(n) * sample
+    return json.loads(r.stdout)
+
+
+def cpu_baseline(threads: int) -> dict | None:
+    """The reference build timed on this host: kiss_fftr + kiss_fftri at N=4096 and N=4000.
+
+    Three forms of the reference at its test flags (-O3 -ffast-math), each one warm-up pass
+    and the median of 5 passes, cfgs allocated before timing:
+      batch   (the reported value) OpenMP parallel for over the batch on `threads` cores, one
+              cfg per thread;
+      serial  the build without OpenMP, one thread;
+      intra   the as-shipped OpenMP build (kiss_fft.c:250-271): one caller, each transform's
+              top-level butterflies split over `threads` threads."""
+    base = os.path.join(ROOT, "oracle", "_ref")
+    exe, exe_serial = os.path.join(base, "cpu_baseline-float"), os.path.join(base, "cpu_baseline_serial-float")
+    if not (os.path.exists(exe) and os.path.exists(exe_serial)):
+        return None
+    forms = (("batch", exe, threads, 16384), ("serial", exe_serial, 1, 2048), ("intra", exe, threads, 2048))
+    res = {}
+    for form, path, th, sample in forms:
+        total_bytes = total_s = 0.0
+        for n in SIZES:
+            for kind in ("r2c", "c2r"):
+                d = _ref_pass(path, kind, n, sample, th, "intra" if form == "intra" else "batch")
+                if d is None:
+                    return None
+                total_s += d["median_s"]
+                total_bytes += alg_bytes(n) * sample
+        res[form] = {"GBps": round(total_bytes / total_s / 1e9, 3), "transforms_per_s": round(4 * sample / total_s, 1),
+                     "threads": th, "signals_per_size_and_direction": sample}
     return {
-        "value": round(total_bytes / total_s / 1e9, 3),
+        "value": res["batch"]["GBps"],
         "unit": "GB/s",
-        "transforms_per_s": round(4 * sample / total_s, 1),
+        "transforms_per_s": res["batch"]["transforms_per_s"],
         "cores": threads,
         "kind": "reference",
-        "sample": f"kiss_fftr + kiss_fftri at N=4096 and N=4000, {sample} signals each, "
-                  f"OpenMP parallel for over the batch ({threads} threads, one cfg per thread), "
-                  "reference sources at -O3 -ffast-math, median of 3 passes after 1 warm-up",
+        "nproc": os.cpu_count(),
+        "cpu_model": _cpu_model(),
+        "sample": f"kiss_fftr + kiss_fftri at N=4096 and N=4000, {forms[0][3]} signals each, OpenMP parallel "
+                  f"for over the batch ({threads} threads: this process's CPU share; one cfg per thread), "
+                  "reference sources at -O3 -ffast-math, median of 5 passes after 1 warm-up",
+        "forms": res,
     }
 
 
@@ -80,6 +112,57 @@ def _time(fn, stream, reps: int) -> float:
     e1.record(stream)
     e1.synchronize()
     return e0.elapsed_time(e1) / 1e3 / reps
+
+
+def ceilings(stream, dev) -> dict | None:
+    """Streaming rates of hand-written 16-byte-lane kernels on this device (libkfa_membench,
+    kissfft_amd/csrc/bench/kfa_membench.hip): read-only, copy, and the real-forward row
+    pattern of the N=4096 batch (65536 rows of 4096 floats in, 2049 complex out).  HIP events
+    on `stream`, best of 3 rounds of 10 launches."""
+    import ctypes
+
+    import torch
+
+    path = os.path.join(ROOT, "kissfft_amd", "lib", "libkfa_membench.so")
+    if not os.path.exists(path):
+        return None
+    mb = ctypes.CDLL(path)
+    vp, sz, ll, ci = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_longlong, ctypes.c_int
+    mb.kfa_mb_read.argtypes = [vp, sz, vp, vp]
+    mb.kfa_mb_copy.argtypes = [vp, vp, sz, vp]
+    mb.kfa_mb_rows.argtypes = [vp, vp, ll, ci, vp]
+    sp = stream.cuda_stream
+    rows_in = BATCH * 4096 * 4
+    rows_out = BATCH * 2049 * 8
+    with torch.cuda.stream(stream):
+        a = torch.rand(rows_in // 4, device=dev)
+        b = torch.empty(rows_out // 4 + 4, device=dev)
+        sink = torch.empty(1 << 18, device=dev)
+    stream.synchronize()
+
+    def rate(fn, nbytes):
+        best = 0.0
+        fn()
+        for _ in range(3):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(10):
+                fn()
+            e1.record(stream)
+            e1.synchronize()
+            best = max(best, nbytes * 10 / (e0.elapsed_time(e1) / 1e3) / 1e9)
+        return round(best, 1)
+
+    out = {
+        "read_GBps": rate(lambda: mb.kfa_mb_read(a.data_ptr(), rows_in, sink.data_ptr(), sp), rows_in),
+        "copy_GBps": rate(lambda: mb.kfa_mb_copy(a.data_ptr(), b.data_ptr(), rows_in, sp), 2 * rows_in),
+        "r2c_rows_GBps": rate(lambda: mb.kfa_mb_rows(a.data_ptr(), b.data_ptr(), BATCH, 4096, sp),
+                              rows_in + rows_out),
+        "unit": "GB/s",
+        "kernels": "hand-written streaming kernels, 16-byte lanes (kissfft_amd/csrc/bench/kfa_membench.hip)",
+    }
+    del a, b, sink
+    return out
 
 
 def other_configs(lib, stream, dev) -> dict:
@@ -291,16 +374,29 @@ def serving(lib, dev, count: int, rate: float, streams: int = 0) -> dict:
                      else lib.fftr_alloc(n, kind == "c2r"))
     L = lib.L
     calls = {"c2c": L.kiss_fft, "c2c_inv": L.kiss_fft, "r2c": L.kiss_fftr, "c2r": L.kiss_fftri}
-    t0 = time.perf_counter()
+    # every single call is timed (each returns with its result valid), and each request's
+    # run of calls; the per-call latency under the same open-loop arrivals is the FIFO
+    # queue of those request durations on one caller thread
+    call_s, req_s = [], []
+    pc = time.perf_counter
+    t0 = pc()
     for r, ob in zip(reqs, obufs):
         key = (r["kind"], r["n"])
         ib, ob1 = W.request_bytes(r["kind"], r["n"], 1)
         fn, cfg = calls[r["kind"]], cfgs[key]
         ip, op = ibufs[key].data_ptr(), ob.data_ptr()
+        tr = pc()
         for b in range(r["batch"]):
+            tc = pc()
             fn(cfg, ip + b * ib, op + b * ob1)
+            call_s.append(pc() - tc)
+        req_s.append(pc() - tr)
     torch.cuda.synchronize()
-    per_call_s = time.perf_counter() - t0
+    per_call_s = pc() - t0
+    done, fifo = 0.0, []
+    for r, d in zip(reqs, req_s):
+        done = max(done, r["t"]) + d
+        fifo.append(done - r["t"])
     same = all(torch.equal(served[i], obufs[i][: served[i].numel()].cpu()) for i in sample)
     srv.close()
     del ibufs, obufs
@@ -308,6 +404,10 @@ def serving(lib, dev, count: int, rate: float, streams: int = 0) -> dict:
 
     def ms(v):
         return round(v * 1e3, 3)
+
+    def q(v, f):
+        v = sorted(v)
+        return v[min(len(v) - 1, int(f * (len(v) - 1) + 0.5))] if v else 0.0
 
     sat_gbps = total_bytes / sat["span_s"] / 1e9
     pc_gbps = total_bytes / per_call_s / 1e9
@@ -325,7 +425,11 @@ def serving(lib, dev, count: int, rate: float, streams: int = 0) -> dict:
                       "span_ms": ms(sat["span_s"]), "p50_ms": ms(sat["p50_s"]), "p99_ms": ms(sat["p99_s"]),
                       "launches": sat["launches"]},
         "per_call": {"GBps": round(pc_gbps, 2), "transforms_per_s": round(transforms / per_call_s, 1),
-                     "seconds": round(per_call_s, 3)},
+                     "seconds": round(per_call_s, 3),
+                     "call_p50_us": round(1e6 * q(call_s, 0.50), 1), "call_p99_us": round(1e6 * q(call_s, 0.99), 1),
+                     "request_p50_ms": ms(q(req_s, 0.50)), "request_p99_ms": ms(q(req_s, 0.99)),
+                     "open_loop_p50_ms": ms(q(fifo, 0.50)), "open_loop_p99_ms": ms(q(fifo, 0.99)),
+                     "open_loop_rate_per_s": rate},
         "speedup_vs_per_call": round(sat_gbps / pc_gbps, 2),
         "served_equals_per_call": bool(same),
     }
@@ -470,6 +574,7 @@ def main() -> int:
             s = ev0.elapsed_time(ev1) / 1e3 / 5
             per_kernel[f"{name}_{n}"] = {"us": round(s * 1e6, 1), "GBps": round(alg_bytes(n) * BATCH / s / 1e9, 1)}
 
+    ceil = ceilings(stream, dev)
     others = {} if args.no_other_configs else other_configs(lib, stream, dev)
     serve = None
     if not args.no_serving:
@@ -485,7 +590,7 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
         try:
             cpu = cpu_baseline(threads)
         except Exception as e:  # the baseline is reported, never required
@@ -525,6 +630,8 @@ def main() -> int:
             "traffic": traffic,
             "alg_bytes_per_launch": kbytes,
             "avg_launch_us": round(kern_s * 1e6, 2),
+            "ceiling": ceil,
+            "frac_of_rows_ceiling": round(achieved / ceil["r2c_rows_GBps"], 4) if ceil else None,
         },
         "cpu_baseline": cpu,
     }

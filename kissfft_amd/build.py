@@ -12,6 +12,7 @@ Layout (kissfft_amd/lib/):
     libkissfft-<type>.so             symlink to the above
     fftutil-<type>                   the stream tool (tools/fftutil.c protocol), linked
                                      against the library; alias fastfilt-<type>
+    libkfa_membench.so               streaming-ceiling kernels for bench.py (not the library)
 
 The host layer is compiled with gcc and without -ffast-math (the fixed-point twiddle
 tables depend on exact FP64 rounding).  All HIP code is compiled with packed-FP32
@@ -147,7 +148,18 @@ def build(datatypes=None, verbose: bool = True) -> list[str]:
         os.symlink(soname, link)
         built.append(real)
         built.append(build_fftutil(dt, verbose))
+    built.append(build_membench(verbose))
     return built
+
+
+def build_membench(verbose: bool = True) -> str:
+    """libkfa_membench.so: the streaming-ceiling kernels bench.py reports beside the FFT
+    kernels (not part of the FFT library)."""
+    out = os.path.join(LIBDIR, "libkfa_membench.so")
+    tmp = out + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, os.path.join(CSRC, "bench", "kfa_membench.hip")], verbose)
+    os.replace(tmp, out)
+    return out
 
 
 def build_fftutil(dt: str, verbose: bool = True) -> str:
