@@ -58,6 +58,22 @@ int KISS_FFT_API kiss_fftndr_batch_async(kiss_fftndr_cfg cfg, const kiss_fft_sca
 int KISS_FFT_API kiss_fftndri_batch_async(kiss_fftndr_cfg cfg, const kiss_fft_cpx *freqdata,
                                           kiss_fft_scalar *timedata, long long howmany, void *stream);
 
+/* kiss_fft_batch_noalloc: kiss_fft_batch_async that allocates nothing.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid allocation).
+ *   - fin / fout must be device memory; NULL `stream` is the null stream.
+ *   - The cfg's device plan and kernel choice must exist on this device: run the same cfg
+ *     once through kiss_fft_batch / kiss_fft_batch_async first.  Otherwise the call fails.
+ *   - Lengths longer than one LDS line (float / Q31 above 8192 points, Q15 above 16384)
+ *     take their temporaries from a static 64 MiB device array: one transform's
+ *     temporary must fit it (float 2^22 points at most); larger batches run in chunks.
+ *     Longer lengths are rejected with an error.
+ *   - Every call on one GPU shares that array: calls that overlap on one GPU race.
+ * Returns 0 or a negative code (kiss_fft_last_error()). */
+int KISS_FFT_API kiss_fft_batch_noalloc(kiss_fft_cfg cfg, const kiss_fft_cpx *fin, int in_stride,
+                                        long long idist, kiss_fft_cpx *fout, long long odist,
+                                        long long howmany, void *stream);
+
 /* Waits for everything enqueued on `stream` (NULL: the calling thread's stream). */
 int KISS_FFT_API kiss_fft_stream_sync(void *stream);
 

@@ -134,6 +134,7 @@ class Library:
             "kiss_fftnd_batch_async": ([vp, vp, vp, ll, vp], ci),
             "kiss_fftndr_batch_async": ([vp, vp, vp, ll, vp], ci),
             "kiss_fftndri_batch_async": ([vp, vp, vp, ll, vp], ci),
+            "kiss_fft_batch_noalloc": ([vp, vp, ci, ll, vp, ll, ll, vp], ci),
             "kiss_fft_stream_sync": ([vp], ci),
             "kiss_fft_last_error": ([], ctypes.c_char_p),
             "kiss_fft_device_available": ([], ci),
@@ -238,6 +239,14 @@ class Library:
             rc = self.L.kiss_fft_batch_async(cfg, _addr(fin), in_stride, idist, _addr(fout), odist, howmany,
                                              _stream_ptr(stream))
         self.check(rc, "kiss_fft_batch")
+
+    def fft_batch_noalloc(self, cfg: Cfg, fin, fout, howmany: int, idist: int, odist: int, in_stride: int = 1,
+                          stream=None) -> None:
+        """kiss_fft_batch_noalloc: device buffers, nothing allocated (the cfg must have run
+        once through fft_batch on this device)."""
+        rc = self.L.kiss_fft_batch_noalloc(cfg, _addr(fin), in_stride, idist, _addr(fout), odist, howmany,
+                                           _stream_ptr(stream) if stream is not None else None)
+        self.check(rc, "kiss_fft_batch_noalloc")
 
     def fftr_batch(self, cfg: Cfg, timedata, freqdata, howmany: int, idist: int, odist: int, stream=None):
         if stream is None:

@@ -1983,6 +1983,81 @@ int kfa_c2c(const kfa_cplan *pl, const void *in, long long istride, long long id
     return c2c_call(pl, in, istride, idist, out, odist, howmany, s, dev, sync);
 }
 
+// ------------------------------------------------------------------------------------
+// The no-allocation form of kfa_c2c: the temporaries of multi-pass lengths (and of
+// overlapping buffers) come from `scratch`, a file-scope device array each GPU holds for
+// the life of the process; every other allocation site refuses (device plans, kernel
+// choices and tables must exist from an earlier allocating call; host buffers would need
+// staging).  The array's device address is looked up once per device with that device
+// current (the host symbol is not a device address).  Every call on a GPU shares that
+// GPU's array: calls that overlap on one GPU race.  The temporaries are written in full
+// before they are read, so nothing needs clearing between calls.
+// ------------------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+static std::mutex g_scratch_mu;
+static std::map<int, void *> g_scratch_ptr;  // per device: the address of `scratch`
+
+static int noalloc_scratch(int dev, void **ptr)
+{
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    if (it == g_scratch_ptr.end()) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(-3, "no-allocation scratch: hipGetSymbolAddress failed: %s", hipGetErrorString(e));
+        it = g_scratch_ptr.emplace(dev, p).first;
+    }
+    *ptr = it->second;
+    return 0;
+}
+
+int kfa_c2c_noalloc(const kfa_cplan *pl, const void *in, long long istride, long long idist, void *out,
+                    long long odist, long long howmany, void *stream)
+{
+    int dev;
+    int rc = current_device(&dev);
+    if (rc)
+        return rc;
+    // NULL is the null stream here: the calling thread's own stream may not exist yet
+    const hipStream_t s = (hipStream_t)stream;
+    void *base = nullptr;
+    if ((rc = noalloc_scratch(dev, &base)))
+        return rc;
+    if (!device_accessible(in) || !device_accessible(out))
+        return fail(-2, "no-allocation call: buffers must be device memory");
+    Workspace ws;
+    ws.base = static_cast<char *>(base);
+    ws.size = kNoAllocScratchBytes;
+    struct Mode {  // the no-allocation state of this thread, restored on every return
+        Workspace *prev_ws;
+        bool prev_noalloc;
+        Mode(Workspace *w) : prev_ws(thread_workspace()), prev_noalloc(thread_noalloc())
+        {
+            thread_workspace() = w;
+            thread_noalloc() = true;
+        }
+        ~Mode()
+        {
+            thread_workspace() = prev_ws;
+            thread_noalloc() = prev_noalloc;
+        }
+    } mode(&ws);
+    // a multi-pass transform must fit its temporary in the array (the batch runs in chunks)
+    {
+        std::shared_ptr<DevPlan> P;
+        if ((rc = build_cplan(pl, dev, P)))
+            return rc;
+        const size_t per = multipass_bytes_per_transform(*P, FK_C2C);
+        if (per && per + 4096 > kNoAllocScratchBytes)
+            return fail(-2, "no-allocation call: nfft=%d needs %zu bytes of scratch per transform; the static "
+                            "scratch holds %zu", pl->nfft, per, kNoAllocScratchBytes);
+    }
+    return c2c_call(pl, in, istride, idist, out, odist, howmany, s, dev, 0);
+}
+
 static int real_common(const kfa_rplan *pl, const void *in, long long idist, void *out, long long odist,
                        long long howmany, void *stream, int sync, int inverse)
 {

@@ -146,6 +146,16 @@ int kiss_fft_batch_async(kiss_fft_cfg cfg, const kiss_fft_cpx *fin, int in_strid
     return kfa_c2c(&pl, fin, in_stride, idist, fout, odist, howmany, stream, 0);
 }
 
+int kiss_fft_batch_noalloc(kiss_fft_cfg cfg, const kiss_fft_cpx *fin, int in_stride, long long idist,
+                           kiss_fft_cpx *fout, long long odist, long long howmany, void *stream)
+{
+    kfa_cplan pl;
+    if (!cfg || !fin || !fout)
+        return -2;
+    kf_cplan_of(cfg, &pl);
+    return kfa_c2c_noalloc(&pl, fin, in_stride, idist, fout, odist, howmany, stream);
+}
+
 int kiss_fft_stream_sync(void *stream)
 {
     return kfa_stream_sync(stream);

@@ -44,6 +44,11 @@ typedef struct {
 KFA_HIDDEN int kfa_c2c(const kfa_cplan *pl, const void *in, long long istride, long long idist,
                        void *out, long long odist, long long howmany, void *stream, int sync);
 
+/* kfa_c2c without any allocation: device buffers only, the plan prepared by an earlier
+ * kfa_c2c on this device, multi-pass temporaries in a static device array. */
+KFA_HIDDEN int kfa_c2c_noalloc(const kfa_cplan *pl, const void *in, long long istride, long long idist,
+                               void *out, long long odist, long long howmany, void *stream);
+
 KFA_HIDDEN int kfa_r2c(const kfa_rplan *pl, const void *in, long long idist, void *out,
                        long long odist, long long howmany, void *stream, int sync);
 
