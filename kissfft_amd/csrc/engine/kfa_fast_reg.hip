@@ -13,6 +13,7 @@
 
 #include "kfa_fast.hpp"
 #include "kfa_fast_reg.hpp"
+#include "kfa_q15w.hpp"
 
 namespace kfa {
 
@@ -72,6 +73,41 @@ static void add_variant(std::vector<FastVariant> &v, int type, const char *name)
         f.name = name;
         f.launch = inv ? &launch_fast<A, KIND, true, T, LPB, DB, FUSE, RS>
                        : &launch_fast<A, KIND, false, T, LPB, DB, FUSE, RS>;
+        v.push_back(f);
+    }
+}
+
+template <bool INV, int T, int LPB, class RS, bool ASM>
+static int launch_wave(const FastArgs &a, hipStream_t s)
+{
+    const long long blocks = (a.nlines + LPB - 1) / LPB;
+    if (blocks <= 0)
+        return 0;
+    hipLaunchKernelGGL((k_q15w<INV, T, LPB, RS, ASM>), dim3((unsigned)blocks), dim3(T * LPB), 0, s, a);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+// Q15 complex lines held by one wavefront (kfa_q15w.hpp): stage exchanges by lane swaps,
+// no LDS; radix 2 / 4 power-of-two schedules
+template <int T, int LPB, bool ASM, int... R>
+static void add_wave(std::vector<FastVariant> &v, const char *name)
+{
+    typedef Radices<TwPad<Q15P>::G, R...> RS;
+    static_assert(T * LPB <= 1024, "workgroup too large");
+    for (int inv = 0; inv < 2; ++inv) {
+        FastVariant f;
+        f.type = KFA_Q15;
+        f.kind = FK_C2C;
+        f.n = RS::N();
+        f.inverse = inv;
+        f.T = T;
+        f.LPB = LPB;
+        f.radices = {R...};
+        f.tw_pad = TwPad<Q15P>::G;
+        f.tw_fmt = TwPad<Q15P>::fmt;
+        f.tw_len = RS::tw_len();
+        f.name = name;
+        f.launch = inv ? &launch_wave<true, T, LPB, RS, ASM> : &launch_wave<false, T, LPB, RS, ASM>;
         v.push_back(f);
     }
 }
@@ -166,6 +202,15 @@ const std::vector<FastVariant> &fast_registry()
         add_variant<F32, FK_C2C, 125, 2, false, false, 5, 5, 5, 4>(*v, KFA_F32, "c2c500_5554_t125x2_sb");
         add_variant<F32, FK_C2C, 192, 1, false, false, 3, 16, 8, 8>(*v, KFA_F32, "c2c3072_31688_t192_sb");
         add_variant<F32, FK_C2C, 256, 1, false, false, 16, 16, 16>(*v, KFA_F32, "c2c4096_161616_t256_sb");
+        // ---- Q15 (reference schedules), wavefront-resident lines: lane-swap exchanges ----
+        add_wave<16, 16, true, 2, 4, 4, 4, 4>(*v, "q15w_c2c512_t16x16_asm");
+        add_wave<32, 8, true, 2, 4, 4, 4, 4>(*v, "q15w_c2c512_t32x8_asm");
+        add_wave<16, 16, false, 2, 4, 4, 4, 4>(*v, "q15w_c2c512_t16x16");
+        add_wave<32, 8, false, 2, 4, 4, 4, 4>(*v, "q15w_c2c512_t32x8");
+        add_wave<16, 8, false, 2, 4, 4, 4, 4>(*v, "q15w_c2c512_t16x8");
+        add_wave<16, 4, false, 2, 4, 4, 4, 4>(*v, "q15w_c2c512_t16x4");
+        add_wave<64, 4, false, 2, 4, 4, 4, 4>(*v, "q15w_c2c512_t64x4");
+        add_wave<32, 4, false, 2, 4, 4, 4, 4>(*v, "q15w_c2c512_t32x4");
         // ---- Q15 (reference schedules), packed-word butterflies ----
         add_variant<Q15P, FK_C2C, 32, 8, false, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c512_t32x8_sb");
         add_variant<Q15P, FK_C2C, 16, 16, false, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c512_t16x16_sb");

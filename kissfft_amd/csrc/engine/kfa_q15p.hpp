@@ -143,15 +143,20 @@ __device__ __forceinline__ u32 fixdiv(u32 c)
 // products first, then all of the packing: a packing instruction right behind the dot
 // product it reads waits a cycle (an s_nop the compiler inserts), one K - 1 products later
 // does not.  Same operations, same bits as fixdiv / cmul one at a time.
+// The two multipliers are held in registers for the K products: the dot product reads one
+// scalar register (the rounding constant) at most, so a multiplier left to the compiler
+// is copied into a vector register before every product.
 template <int P, int K>
 __device__ __forceinline__ void fixdiv_n(u32 (&x)[K], const u32 *c)
 {
     constexpr u32 K2 = 2 * (32767 / P);
+    u32 klo = K2, khi = K2 << 16;
+    asm("" : "+v"(klo), "+v"(khi));
     int re[K], im[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        re[k] = dot_h(c[k], K2);
-        im[k] = dot_h(c[k], K2 << 16);
+        re[k] = dot_h(c[k], klo);
+        im[k] = dot_h(c[k], khi);
     }
 #pragma unroll
     for (int k = 0; k < K; ++k)

@@ -1,0 +1,394 @@
+// kfa_q15w.hpp -- Q15 lines held by one wavefront: the stage exchanges are lane swaps.
+//
+// The radix-2 / radix-4 butterflies of kf_bfly2 / kf_bfly4 (kiss_fft.c:15-84, packed forms
+// of kfa_q15p.hpp, the reference's rounding) on a power-of-two line of N = T * R points held
+// by T lanes of one wavefront, R packed Q15 words per lane.  A point's index is spread over
+// the lane id (log2 T bits) and the register index (log2 R bits).  A radix-2^c stage needs
+// the c index bits of its butterfly inputs in register bits (the butterfly's inputs in one
+// lane); Stockham's data movement between stages is only a relabelling of index bits, so
+// the one exchange a stage may need is a swap of a lane bit with a register bit:
+//   lane bit 0..3   DPP row shifts by 2^k (v_mov_dpp row_shl / row_shr) and a select,
+//                   one pair of instructions per register pair
+//   lane bit 4      v_permlane16_swap (rows of 16 lanes exchanged between two registers)
+//   lane bit 5      v_permlane32_swap
+// There is no LDS traffic and no barrier: lines never leave their wavefront.  A bit that
+// leaves the registers is always one the remaining stages never butterfly (a bit above
+// the stage's butterfly bits), and a compile-time plan (WavePlan) picks the swaps and
+// tracks which index bit every lane and register bit holds.  Loads read 16-byte-aligned
+// runs of consecutive points per register; the final swaps put the output's low index bits
+// on lanes, so every store of a register covers T consecutive outputs.
+//
+// Fixed point keeps every rounding of the reference: the butterflies are the reference's
+// operation order with the reference's twiddle table entries (the stage table), only the
+// positions of the values move.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <utility>
+
+#include "kfa_fast.hpp"
+#include "kfa_q15p.hpp"
+
+namespace kfa {
+
+constexpr int ilog2(int v)
+{
+    int b = 0;
+    while ((1 << b) < v)
+        ++b;
+    return b;
+}
+
+// The swap plan of schedule RS on lines of T lanes.
+template <class RS, int T>
+struct WavePlan {
+    static constexpr int N = RS::N();
+    static constexpr int K = RS::K;
+    static constexpr int NBITS = ilog2(N);
+    static constexpr int TB = ilog2(T);
+    static constexpr int RB = NBITS - TB;
+    static constexpr int R = 1 << RB;
+    static_assert((1 << NBITS) == N && (1 << TB) == T, "power-of-two lines and lanes");
+    static_assert(RB >= 2, "a radix-4 stage needs two register bits");
+
+    struct Data {
+        int nswap[K + 1];       // swaps before stage s (s == K: after the last stage)
+        int swap_lane[K + 1][4];
+        int swap_reg[K + 1][4];
+        int lane_lab[K + 1][6];  // index bit held by lane bit i when stage s computes
+        int reg_lab[K + 1][8];   // ... and by register bit j (s == K: output index bits)
+    };
+
+    static constexpr Data make()
+    {
+        Data d{};
+        int lane[6] = {0, 0, 0, 0, 0, 0}, reg[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int i = 0; i < TB; ++i)
+            lane[i] = i;  // loads: lane bits are the low index bits
+        for (int j = 0; j < RB; ++j)
+            reg[j] = TB + j;
+        for (int s = 0; s < K; ++s) {
+            const int c = ilog2(RS::p(s));
+            const int a = ilog2(N / (RS::p(s) * RS::m(s)));  // log2 of the stage's stride P
+            int ns = 0;
+            for (int b = a; b < a + c; ++b) {
+                for (int i = 0; i < TB; ++i) {
+                    if (lane[i] != b)
+                        continue;
+                    // out goes the register bit with the lowest label above the butterfly
+                    // bits (labels above a + c are never butterflied again)
+                    int best = -1;
+                    for (int j = 0; j < RB; ++j)
+                        if (reg[j] >= a + c && (best < 0 || reg[j] < reg[best]))
+                            best = j;
+                    d.swap_lane[s][ns] = i;
+                    d.swap_reg[s][ns] = best;
+                    ++ns;
+                    const int t = lane[i];
+                    lane[i] = reg[best];
+                    reg[best] = t;
+                }
+            }
+            d.nswap[s] = ns;
+            for (int i = 0; i < TB; ++i)
+                d.lane_lab[s][i] = lane[i];
+            for (int j = 0; j < RB; ++j)
+                d.reg_lab[s][j] = reg[j];
+            // Stockham relabelling: butterfly bits go to the top, the bits above shift down
+            auto rel = [&](int x) { return x < a ? x : (x < a + c ? NBITS - c + (x - a) : x - c); };
+            for (int i = 0; i < TB; ++i)
+                lane[i] = rel(lane[i]);
+            for (int j = 0; j < RB; ++j)
+                reg[j] = rel(reg[j]);
+        }
+        // output: the low TB index bits on lanes
+        int ns = 0;
+        for (int i = 0; i < TB; ++i) {
+            if (lane[i] < TB)
+                continue;
+            int best = -1;
+            for (int j = 0; j < RB; ++j)
+                if (reg[j] < TB && best < 0)
+                    best = j;
+            d.swap_lane[K][ns] = i;
+            d.swap_reg[K][ns] = best;
+            ++ns;
+            const int t = lane[i];
+            lane[i] = reg[best];
+            reg[best] = t;
+        }
+        d.nswap[K] = ns;
+        for (int i = 0; i < TB; ++i)
+            d.lane_lab[K][i] = lane[i];
+        for (int j = 0; j < RB; ++j)
+            d.reg_lab[K][j] = reg[j];
+        return d;
+    }
+    static constexpr Data d = make();
+
+    static constexpr bool valid()
+    {
+        for (int s = 0; s <= K; ++s)
+            if (d.nswap[s] > 4)
+                return false;
+        for (int s = 0; s <= K; ++s)
+            for (int k = 0; k < d.nswap[s]; ++k)
+                if (d.swap_reg[s][k] < 0)
+                    return false;
+        return true;
+    }
+    static_assert(valid(), "no swap plan for this schedule");
+};
+
+namespace q15w {
+
+typedef unsigned u32;
+
+// Lanes whose bit L is clear (L < 4), as a wave-wide mask.
+template <int L>
+constexpr unsigned long long lo_lanes()
+{
+    unsigned long long m = 0;
+    for (int l = 0; l < 64; ++l)
+        if (!((l >> L) & 1))
+            m |= 1ULL << l;
+    return m;
+}
+
+// Lane bit L (< 4) swapped with a register bit for two register pairs (A0, B0), (A1, B1):
+// each new value is one v_cndmask_b32 with a DPP source (the partner lane's value) and the
+// lane mask in VCC -- na = lo ? A : B[lane - 2^L], nb = hi ? B : A[lane + 2^L].  The two
+// wait states a DPP read needs after a vector write of its source come first; bound_ctrl:0
+// makes lanes whose partner is outside the row read 0 (they select their own value).
+template <int L>
+__device__ __forceinline__ void dpp_swap2(u32 &A0, u32 &B0, u32 &A1, u32 &B1)
+{
+    static_assert(L >= 0 && L < 4, "DPP row shifts reach lane bits 0-3");
+    u32 na0, na1, nb0, nb1;
+#define KFA_Q15W_SWAP_ASM(SH)                                                                          \
+    asm("s_mov_b64 vcc, %8\n\t"                                                                           \
+        "s_nop 1\n\t"                                                                                     \
+        "v_cndmask_b32_dpp %0, %5, %4, vcc row_shr:" #SH " row_mask:0xf bank_mask:0xf bound_ctrl:0\n\t"  \
+        "v_cndmask_b32_dpp %1, %7, %6, vcc row_shr:" #SH " row_mask:0xf bank_mask:0xf bound_ctrl:0\n\t"  \
+        "s_not_b64 vcc, vcc\n\t"                                                                          \
+        "v_cndmask_b32_dpp %2, %4, %5, vcc row_shl:" #SH " row_mask:0xf bank_mask:0xf bound_ctrl:0\n\t"  \
+        "v_cndmask_b32_dpp %3, %6, %7, vcc row_shl:" #SH " row_mask:0xf bank_mask:0xf bound_ctrl:0"        \
+        : "=&v"(na0), "=&v"(na1), "=&v"(nb0), "=&v"(nb1)                                                  \
+        : "v"(A0), "v"(B0), "v"(A1), "v"(B1), "s"(lo_lanes<L>())                                          \
+        : "vcc")
+    if constexpr (L == 0)
+        KFA_Q15W_SWAP_ASM(1);
+    else if constexpr (L == 1)
+        KFA_Q15W_SWAP_ASM(2);
+    else if constexpr (L == 2)
+        KFA_Q15W_SWAP_ASM(4);
+    else
+        KFA_Q15W_SWAP_ASM(8);
+#undef KFA_Q15W_SWAP_ASM
+    A0 = na0;
+    A1 = na1;
+    B0 = nb0;
+    B1 = nb1;
+}
+
+// Swaps lane bit L with register bit J across the R registers of a lane.
+template <int L, int J, int R, bool ASM>
+__device__ __forceinline__ void swap_bit(u32 (&v)[R], int lane)
+{
+    if constexpr (ASM && L < 4) {
+        // pairs (x, x | 2^J) taken two at a time
+        constexpr int NP = R / 2;
+        static_assert(NP % 2 == 0, "an even number of register pairs");
+        int xs[NP] = {};
+        int k = 0;
+        for (int x = 0; x < R; ++x)
+            if (!(x & (1 << J)))
+                xs[k++] = x;
+#pragma unroll
+        for (int i = 0; i < NP; i += 2)
+            dpp_swap2<L>(v[xs[i]], v[xs[i] | (1 << J)], v[xs[i + 1]], v[xs[i + 1] | (1 << J)]);
+        return;
+    }
+#pragma unroll
+    for (int x = 0; x < R; ++x) {
+        if (x & (1 << J))
+            continue;
+        u32 &A = v[x];
+        u32 &B = v[x | (1 << J)];
+        if constexpr (L == 4) {
+            // rows 1 / 3 of A trade places with rows 0 / 2 of B
+            const auto r = __builtin_amdgcn_permlane16_swap(A, B, false, false);
+            A = r[0];
+            B = r[1];
+        } else if constexpr (L == 5) {
+            const auto r = __builtin_amdgcn_permlane32_swap(A, B, false, false);
+            A = r[0];
+            B = r[1];
+        } else {
+            // lanes with bit L set take B of lane - 2^L (row_shr), the others A of lane + 2^L
+            // (row_shl); both partners sit in the lane's own row of 16.  Written as selects
+            // whose moved operand is the false one, so each folds into one v_cndmask with a
+            // DPP source: one instruction per new value, no copies.
+            const bool hi = (lane >> L) & 1;
+            const u32 fromB = (u32)__builtin_amdgcn_mov_dpp((int)B, 0x110 + (1 << L), 0xf, 0xf, true);
+            const u32 fromA = (u32)__builtin_amdgcn_mov_dpp((int)A, 0x100 + (1 << L), 0xf, 0xf, true);
+            const u32 na = !hi ? A : fromB;
+            const u32 nb = hi ? B : fromA;
+            A = na;
+            B = nb;
+        }
+    }
+}
+
+template <class PL, int S, int K, int R, bool ASM>
+__device__ __forceinline__ void swaps(u32 (&v)[R], int lane)
+{
+    if constexpr (K < PL::d.nswap[S]) {
+        swap_bit<PL::d.swap_lane[S][K], PL::d.swap_reg[S][K], R, ASM>(v, lane);
+        swaps<PL, S, K + 1, R, ASM>(v, lane);
+    }
+}
+
+// value of the index bits that register index x contributes to a field starting at bit
+// `lo` of width `w` (stage labels of stage S)
+template <class PL, int S>
+__device__ __forceinline__ constexpr int reg_field(int x, int lo, int w)
+{
+    int f = 0;
+    for (int j = 0; j < PL::RB; ++j) {
+        const int lab = PL::d.reg_lab[S][j];
+        if (lab >= lo && lab < lo + w && ((x >> j) & 1))
+            f |= 1 << (lab - lo);
+    }
+    return f;
+}
+
+template <class PL, int S>
+__device__ __forceinline__ int lane_field(int lane, int lo, int w)
+{
+    int f = 0;
+#pragma unroll
+    for (int i = 0; i < PL::TB; ++i) {
+        const int lab = PL::d.lane_lab[S][i];
+        if (lab >= lo && lab < lo + w)
+            f |= ((lane >> i) & 1) << (lab - lo);
+    }
+    return f;
+}
+
+// Stage S: swaps, then every butterfly of the lane (its inputs are the registers whose
+// butterfly bits vary; outputs replace them in place), twiddle rows u from the stage table.
+template <bool INV, class RS, class PL, int S, bool ASM>
+__device__ __forceinline__ void stage(u32 (&v)[PL::R], const uint2 *__restrict__ tab, int lane)
+{
+    constexpr int R = PL::R;
+    swaps<PL, S, 0, R, ASM>(v, lane);
+    constexpr int p = RS::p(S);
+    constexpr int c = ilog2(p);
+    constexpr int m = RS::m(S);
+    constexpr int a = ilog2(RS::N() / (p * m));
+    constexpr int PR = RS::PR(S);
+    constexpr int TWO = RS::tw_off(S);
+    // register bit positions of the butterfly bits a, a+1
+    constexpr int rb0 = [] {
+        for (int j = 0; j < PL::RB; ++j)
+            if (PL::d.reg_lab[S][j] == a)
+                return j;
+        return -1;
+    }();
+    constexpr int rb1 = [] {
+        for (int j = 0; j < PL::RB; ++j)
+            if (PL::d.reg_lab[S][j] == a + 1)
+                return j;
+        return -1;
+    }();
+    static_assert(rb0 >= 0 && (c == 1 || rb1 >= 0), "butterfly bits not in registers");
+    constexpr int mask = (1 << rb0) | (c == 2 ? (1 << rb1) : 0);
+    const int u_lane = lane_field<PL, S>(lane, a + c, PL::NBITS - a - c);
+#pragma unroll
+    for (int x = 0; x < R; ++x) {
+        if (x & mask)
+            continue;
+        const unsigned u = (unsigned)u_lane + (unsigned)reg_field<PL, S>(x, a + c, PL::NBITS - a - c);
+        const uint2 *trow = tab + (TWO + u * PR);
+        if constexpr (c == 1) {
+            uint2 w[1] = {trow[0]};
+            u32 y[2] = {v[x], v[x | (1 << rb0)]};
+            q15p::bfly2(y, w);
+            v[x] = y[0];
+            v[x | (1 << rb0)] = y[1];
+        } else {
+            uint2 w[3] = {trow[0], trow[1], trow[2]};
+            // input r = r0 + 2 r1 sits at register x | r0 << rb0 | r1 << rb1
+            constexpr int o1 = 1 << rb0, o2 = 1 << rb1;
+            u32 y[4] = {v[x], v[x | o1], v[x | o2], v[x | o1 | o2]};
+            q15p::bfly4<INV>(y, w);
+            v[x] = y[0];
+            v[x | o1] = y[1];
+            v[x | o2] = y[2];
+            v[x | o1 | o2] = y[3];
+        }
+    }
+}
+
+template <bool INV, class RS, class PL, bool ASM, int... S>
+__device__ __forceinline__ void stages(std::integer_sequence<int, S...>, u32 (&v)[PL::R], const uint2 *tab, int lane)
+{
+    (stage<INV, RS, PL, S, ASM>(v, tab, lane), ...);
+}
+
+}  // namespace q15w
+
+// Q15 c2c lines of the power-of-two schedule RS, T lanes per line, LPB lines per workgroup.
+// ASM: the lane-bit 0-3 swaps as hand-placed v_cndmask_b32 with DPP sources (else the
+// compiler's moves and selects).
+template <bool INV, int T, int LPB, class RS, bool ASM>
+__global__ void __launch_bounds__(T *LPB) k_q15w(FastArgs a)
+{
+    typedef WavePlan<RS, T> PL;
+    typedef q15w::u32 u32;
+    constexpr int R = PL::R;
+    static_assert(64 % T == 0 || T % 64 == 0, "lines must not straddle wavefronts");
+    static_assert(T <= 64, "one wavefront per line at most");
+    const int line = threadIdx.x / T;
+    const int lane = threadIdx.x - line * T;
+    const long long gl = (long long)blockIdx.x * LPB + line;
+    const bool valid = gl < a.nlines;
+    const q15x2 *gin;
+    q15x2 *gout;
+    // lines past the end compute on the last line's data and store nothing
+    line_base<q15x2>(a, valid ? gl : a.nlines - 1, gin, gout);
+    const uint2 *tab = reinterpret_cast<const uint2 *>(__builtin_assume_aligned(a.stw, 16));
+    const long long ie = a.in_el, oe = a.out_el;
+    const bool unit = ie == 1 && oe == 1;
+    u32 v[R];
+    // index x = lane + T * register
+    if (unit) {
+#pragma unroll
+        for (int x = 0; x < R; ++x)
+            v[x] = __builtin_bit_cast(u32, hbm_ld(gin + lane + T * x));
+    } else {
+#pragma unroll
+        for (int x = 0; x < R; ++x)
+            v[x] = __builtin_bit_cast(u32, hbm_ld(gin + (long long)(lane + T * x) * ie));
+    }
+    q15w::stages<INV, RS, PL, ASM>(std::make_integer_sequence<int, RS::K>{}, v, tab, lane);
+    q15w::swaps<PL, RS::K, 0, R, ASM>(v, lane);
+    if (!valid)
+        return;
+    const int y_lane = q15w::lane_field<PL, RS::K>(lane, 0, PL::NBITS);
+    if (unit) {
+#pragma unroll
+        for (int x = 0; x < R; ++x)
+            hbm_st(gout + y_lane + q15w::reg_field<PL, RS::K>(x, 0, PL::NBITS),
+                   __builtin_bit_cast(q15x2, v[x]));
+    } else {
+#pragma unroll
+        for (int x = 0; x < R; ++x)
+            hbm_st(gout + (long long)(y_lane + q15w::reg_field<PL, RS::K>(x, 0, PL::NBITS)) * oe,
+                   __builtin_bit_cast(q15x2, v[x]));
+    }
+}
+
+}  // namespace kfa
