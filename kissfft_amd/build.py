@@ -40,7 +40,10 @@ DATATYPES = {
     "int32_t": ["-DFIXED_POINT=32"],
 }
 HOST_SOURCES = ["kf_core.c", "kf_real.c", "kf_nd.c", "kf_ndr.c", "kf_cache.c", "kf_serve.c", "kf_multi.c", "kf_fastfir.c"]
-ENGINE_SOURCES = ["kfa_engine.hip", "kfa_fast_reg.hip", "kfa_tile_reg.hip", "kfa_serve.hip", "kfa_multi.hip", "kfa_conv.hip"]
+ENGINE_SOURCES = ["kfa_engine.hip", "kfa_fast_reg.hip", "kfa_tile_reg.hip", "kfa_serve.hip", "kfa_multi.hip", "kfa_conv.hip",
+                  "kfa_rtc.hip"]
+# headers the runtime compiler needs for k_fast (kfa_rtc.hip), embedded in the library
+RTC_HEADERS = ["kfa_fast.hpp", "kfa_arith.hpp", "kfa_lines.hpp", "kfa_q15p.hpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CC = os.environ.get("CC", "gcc")
@@ -128,6 +131,7 @@ def build(datatypes=None, verbose: bool = True) -> list[str]:
         nkernels = sum(f.result() for f in [pool.submit(_run_engine, c, verbose) for c in cmds])
     if verbose:
         print(f"{nkernels} kernels; none but the general line kernels use scratch memory", flush=True)
+    engine_objs.append(build_rtc_headers(objdir, verbose))
     built = []
     for dt in datatypes:
         host_objs = []
@@ -140,7 +144,7 @@ def build(datatypes=None, verbose: bool = True) -> list[str]:
         real = os.path.join(LIBDIR, soname)
         tmp = real + ".tmp"
         _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *host_objs, *engine_objs,
-              f"-Wl,-soname,{soname}", "-Wl,-Bsymbolic", "-lpthread", "-lm"], verbose)
+              f"-Wl,-soname,{soname}", "-Wl,-Bsymbolic", "-lpthread", "-lm", "-ldl"], verbose)
         os.replace(tmp, real)
         link = lib_path(dt)
         if os.path.lexists(link):
@@ -150,6 +154,30 @@ def build(datatypes=None, verbose: bool = True) -> list[str]:
         built.append(build_fftutil(dt, verbose))
     built.append(build_membench(verbose))
     return built
+
+
+def build_rtc_headers(objdir: str, verbose: bool = True) -> str:
+    """kfa_rtc_headers.o: the kernel headers as string constants, for the runtime compiler
+    (kfa_rtc.hip compiles k_fast for lengths outside the built-in registry)."""
+    src = os.path.join(objdir, "kfa_rtc_headers.cpp")
+    lines = ['// generated by kissfft_amd/build.py from kissfft_amd/csrc/engine: do not edit',
+             f'extern "C" const int kfa_rtc_nheaders = {len(RTC_HEADERS)};',
+             'extern "C" const char *const kfa_rtc_header_names[] = {' +
+             ", ".join(f'"{h}"' for h in RTC_HEADERS) + '};',
+             'extern "C" const char *const kfa_rtc_header_srcs[] = {']
+    for h in RTC_HEADERS:
+        with open(os.path.join(CSRC, "engine", h)) as f:
+            text = f.read()
+        # one C string literal per header, split into short pieces
+        pieces = [text[i:i + 2000] for i in range(0, len(text), 2000)] or [""]
+        lines.append("    " + "\n".join(
+            '"' + p.replace("\\", "\\\\").replace('"', '\\"').replace("\n", "\\n") + '"' for p in pieces) + ",")
+    lines.append("};")
+    with open(src, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    obj = os.path.join(objdir, "kfa_rtc_headers.o")
+    _run(["g++", "-O0", "-fPIC", "-fvisibility=hidden", "-c", src, "-o", obj], verbose)
+    return obj
 
 
 def build_membench(verbose: bool = True) -> str:

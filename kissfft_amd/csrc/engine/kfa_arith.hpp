@@ -15,8 +15,17 @@
 // (a rounded product or a shift).  The float butterflies follow the same formulas.
 #pragma once
 
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#else  // compiled at run time (kfa_rtc.hip): the runtime compiler's own integer types
+using __hip_internal::int16_t;
+using __hip_internal::int32_t;
+using __hip_internal::int64_t;
+using __hip_internal::uint16_t;
+using __hip_internal::uint32_t;
+using __hip_internal::uint64_t;
+#endif
 
 namespace kfa {
 

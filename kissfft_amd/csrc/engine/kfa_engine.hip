@@ -1161,12 +1161,16 @@ int choose_fast(DevPlan &P, int kind, hipStream_t s, const FastChoice **out, lon
             continue;  // fixed point must keep the reference schedule
         cand.push_back(&v);
     }
+    if (thread_noalloc())
+        return noalloc_refuse("the kernel choice");
+    // lengths without a built-in variant: compiled at run time (not while the caller's
+    // stream is being captured: such a plan keeps the general kernel)
+    if (cand.empty() && !stream_capturing(s))
+        (void)rtc_candidates(P, kind, cand);
     if (cand.empty()) {
         fc.decided = true;
         return 0;
     }
-    if (thread_noalloc())
-        return noalloc_refuse("the kernel choice");
     std::vector<DevBuf> tabs(cand.size());
     for (size_t i = 0; i < cand.size(); ++i) {
         if ((long long)(cand[i]->tw_len) <= 0)
@@ -1184,16 +1188,27 @@ int choose_fast(DevPlan &P, int kind, hipStream_t s, const FastChoice **out, lon
                 forced = true;
             }
     }
+    // time every candidate on a scratch batch large enough to stream from HBM, as large as
+    // the call that asks (between 256 MiB and 1 GiB per side, beyond the last-level cache):
+    // the ranking of variants depends on the batch size.  Earlier decisions for the same
+    // device, length, kind and timing batch come from the persisted table.
+    const size_t cb = cpx_bytes(P.type);
+    const long long per_line = (long long)cb * (P.n + 1);
+    long long tune_lines = std::max(lines_hint, (256LL << 20) / per_line);
+    tune_lines = std::max(64LL, std::min(tune_lines, (1LL << 30) / per_line));
+    const bool tune = !forced && cand.size() > 1 && env_flag("KISSFFT_AMD_TUNE", true);
+    const std::string tkey = tune ? tuned_key(P, kind, tune_lines) : std::string();
+    std::string tuned;
+    bool from_table = false;
+    if (tune && tuned_lookup(tkey, tuned))
+        for (size_t i = 0; i < cand.size(); ++i)
+            if (tuned == cand[i]->name) {
+                best = i;
+                from_table = true;
+            }
     hipStream_t ts;
-    if (!forced && cand.size() > 1 && env_flag("KISSFFT_AMD_TUNE", true) && !stream_capturing(s) &&
-        tune_stream(&ts) == 0) {
-        // time every candidate on a scratch batch large enough to stream from HBM, as large
-        // as the call that asks (between 256 MiB and 1 GiB per side, beyond the last-level
-        // cache): the ranking of variants depends on the batch size
-        const size_t cb = cpx_bytes(P.type);
-        const long long per_line = (long long)cb * (P.n + 1);
-        long long lines = std::max(lines_hint, (256LL << 20) / per_line);
-        lines = std::max(64LL, std::min(lines, (1LL << 30) / per_line));
+    if (tune && !from_table && !stream_capturing(s) && tune_stream(&ts) == 0) {
+        long long lines = tune_lines;
         for (; lines >= 64; lines /= 4) {
             const size_t bytes = (size_t)per_line * (size_t)lines;
             const size_t rtb = cb * (size_t)(P.n / 2 + 1);
@@ -1218,8 +1233,9 @@ int choose_fast(DevPlan &P, int kind, hipStream_t s, const FastChoice **out, lon
             };
             char what[64];
             snprintf(what, sizeof what, "n=%d kind=%d", P.n, kind);
-            best = time_candidates(cand.size(), ts, [&](size_t i) { return cand[i]->launch(args_for(i), ts); },
+            best = time_candidates(cand.size(), ts, [&](size_t i) { return fast_launch(*cand[i], args_for(i), ts); },
                                    what);
+            tuned_store(tkey, cand[best]->name);
             break;
         }
     }
@@ -1491,7 +1507,7 @@ int run_c2c(DevPlan &P, const void *in, long long istride, long long idist, void
             a.out_hi = odist;
             a.out_el = ostride;
             ++g_launches;
-            return fc->v->launch(a, s) ? fail(-3, "fast kernel launch failed") : 0;
+            return fast_launch(*fc->v, a, s) ? fail(-3, "fast kernel launch failed") : 0;
         }
         std::shared_ptr<const BlueChoice> bc;
         if ((rc = blue_choice(P, bc)))
@@ -1850,7 +1866,7 @@ int run_real_rows(DevPlan &H, const RealTables &R, int inverse, const void *in, 
         a.out_hi = out_hi;
         a.out_el = 1;
         ++g_launches;
-        return fc->v->launch(a, s) ? fail(-3, "fast kernel launch failed") : 0;
+        return fast_launch(*fc->v, a, s) ? fail(-3, "fast kernel launch failed") : 0;
     }
     LineLaunch ll{};
     ll.in = in;

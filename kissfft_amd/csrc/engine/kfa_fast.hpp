@@ -17,7 +17,17 @@
 // reference-order butterflies of kfa_arith.hpp.
 #pragma once
 
+#ifndef __HIPCC_RTC__
 #include <utility>
+#else  // compiled at run time (kfa_rtc.hip): no standard library; the sequence type only
+namespace std {
+template <class T, T... I>
+struct integer_sequence {
+};
+template <class T, T N>
+using make_integer_sequence = __make_integer_seq<integer_sequence, T, N>;
+}  // namespace std
+#endif
 
 #include "kfa_arith.hpp"
 #include "kfa_lines.hpp"

@@ -24,9 +24,12 @@ struct FastVariant {
     int tw_len;   // entries of the per-stage twiddle table
     const char *name;
     int (*launch)(const FastArgs &, hipStream_t);
+    const void *rtc = nullptr;  // runtime-compiled variant (kfa_rtc.hip): launched through it
 };
 
 const std::vector<FastVariant> &fast_registry();
+// launches variant v (built in or runtime-compiled)
+int fast_launch(const FastVariant &v, const FastArgs &a, hipStream_t s);
 
 }  // namespace kfa
 

@@ -7,6 +7,7 @@
 #include <atomic>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "../kfa_engine.h"
@@ -137,6 +138,13 @@ int build_rtab(const kfa_rplan *hp, int dev, std::shared_ptr<RealTables> &out);
 // the fast variant for one transform kind of a plan (decided once per plan and kind, timed
 // on a batch sized from lines_hint, the number of lines of the call that asks)
 int choose_fast(DevPlan &P, int kind, hipStream_t s, const FastChoice **out, long long lines_hint = 0);
+
+// runtime-compiled variants of a plan's length and kind (kfa_rtc.hip), when none is built in
+int rtc_candidates(const DevPlan &P, int kind, std::vector<const FastVariant *> &out);
+// the persisted table of tuned choices (kfa_rtc.hip): key of a decision, lookup, store
+std::string tuned_key(const DevPlan &P, int kind, long long tune_lines);
+bool tuned_lookup(const std::string &key, std::string &name);
+void tuned_store(const std::string &key, const std::string &name);
 
 // launches between device buffers, enqueued on s
 int run_c2c(DevPlan &P, const void *in, long long istride, long long idist, void *out, long long ostride,

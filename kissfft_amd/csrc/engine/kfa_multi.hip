@@ -137,7 +137,7 @@ static int run_multi(Multi &M, hipStream_t s)
     // groups are sorted largest first at creation
     for (size_t i = 0; i < M.groups.size() && rc == 0; ++i) {
         ++g_launches;
-        if (M.groups[i].v->launch(M.groups[i].a, pick()))
+        if (fast_launch(*M.groups[i].v, M.groups[i].a, pick()))
             rc = fail(-3, "fast kernel launch failed");
     }
     for (size_t i = 0; i < M.blue.size() && rc == 0; ++i)

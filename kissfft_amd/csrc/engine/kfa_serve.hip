@@ -182,7 +182,7 @@ int Server::launch_group(std::vector<std::shared_ptr<ServeReq>> &grp, hipStream_
         a.out = grp[0]->d_out;
         a.nlines = line;
         ++g_launches;
-        return fc->v->launch(a, s) ? fail(-3, "fast kernel launch failed") : 0;
+        return fast_launch(*fc->v, a, s) ? fail(-3, "fast kernel launch failed") : 0;
     }
     // no compiled kernel for this key (or a multi-pass length, or N-D): one launch per request
     for (auto &r : grp) {
