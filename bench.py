@@ -128,8 +128,8 @@ def ceilings(stream, dev) -> dict | None:
         return None
     mb = ctypes.CDLL(path)
     vp, sz, ll, ci = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_longlong, ctypes.c_int
-    mb.kfa_mb_read.argtypes = [vp, sz, vp, vp]
-    mb.kfa_mb_copy.argtypes = [vp, vp, sz, vp]
+    mb.kfa_mb_read.argtypes = [vp, sz, vp, vp, ci, ci]
+    mb.kfa_mb_copy.argtypes = [vp, vp, sz, vp, ci, ci]
     mb.kfa_mb_rows.argtypes = [vp, vp, ll, ci, vp]
     sp = stream.cuda_stream
     rows_in = BATCH * 4096 * 4
@@ -137,7 +137,7 @@ def ceilings(stream, dev) -> dict | None:
     with torch.cuda.stream(stream):
         a = torch.rand(rows_in // 4, device=dev)
         b = torch.empty(rows_out // 4 + 4, device=dev)
-        sink = torch.empty(1 << 18, device=dev)
+        sink = torch.empty(1 << 20, device=dev)  # 4 MiB: 16 bytes for each of up to 256K workgroups
     stream.synchronize()
 
     def rate(fn, nbytes):
@@ -153,9 +153,13 @@ def ceilings(stream, dev) -> dict | None:
             best = max(best, nbytes * 10 / (e0.elapsed_time(e1) / 1e3) / 1e9)
         return round(best, 1)
 
+    # the best of a few launch shapes (workgroups per CU, 16-byte loads in flight per thread)
+    shapes = [(16, 4), (32, 4), (128, 4)]
     out = {
-        "read_GBps": rate(lambda: mb.kfa_mb_read(a.data_ptr(), rows_in, sink.data_ptr(), sp), rows_in),
-        "copy_GBps": rate(lambda: mb.kfa_mb_copy(a.data_ptr(), b.data_ptr(), rows_in, sp), 2 * rows_in),
+        "read_GBps": max(rate(lambda g=g, u=u: mb.kfa_mb_read(a.data_ptr(), rows_in, sink.data_ptr(), sp, g, u),
+                              rows_in) for g, u in shapes),
+        "copy_GBps": max(rate(lambda g=g, u=u: mb.kfa_mb_copy(a.data_ptr(), b.data_ptr(), rows_in, sp, g, u),
+                              2 * rows_in) for g, u in shapes),
         "r2c_rows_GBps": rate(lambda: mb.kfa_mb_rows(a.data_ptr(), b.data_ptr(), BATCH, 4096, sp),
                               rows_in + rows_out),
         "unit": "GB/s",
@@ -208,6 +212,21 @@ def other_configs(lib, stream, dev) -> dict:
     t = _time(lambda: l31.fft_batch(c31, x31, y31, B, 512, 512, stream=stream), stream, 5)
     rec("c2c_q31_512_x524288", t, B * 16 * 512, B)
     del x31, y31
+
+    # a length outside the built-in registry (3000 = 2^3 * 3 * 5^3): compiled at run time on
+    # first use (plus the timing of its thread shapes); the first call is reported apart
+    with torch.cuda.stream(stream):
+        xr = torch.rand(65536, 3000, 2, device=dev) * 2 - 1
+        yr = torch.empty_like(xr)
+    stream.synchronize()
+    cr = lib.fft_alloc(3000)
+    t0 = time.perf_counter()
+    lib.fft_batch(cr, xr, yr, 65536, 3000, 3000, stream=stream)
+    stream.synchronize()
+    first = time.perf_counter() - t0
+    t = _time(lambda: lib.fft_batch(cr, xr, yr, 65536, 3000, 3000, stream=stream), stream, 10)
+    rec("c2c_float_3000_x65536_runtime_compiled", t, 65536 * 16 * 3000, 65536, first_call_ms=round(first * 1e3, 1))
+    del xr, yr
 
     # 3-D complex float 64x96x120, 16 transforms
     dims = (64, 96, 120)
@@ -435,6 +454,98 @@ def serving(lib, dev, count: int, rate: float, streams: int = 0) -> dict:
     }
 
 
+def _profile_setup(name, lib, stream, dev):
+    """(call, algorithmic bytes per call) of one named workload, inputs resident in HBM."""
+    import torch
+
+    import kissfft_amd
+    with torch.cuda.stream(stream):
+        if name in ("c2c_1m", "c2c_64k"):
+            n, b = (1 << 20, 256) if name == "c2c_1m" else (65536, 4096)
+            x = torch.rand(b, n, 2, device=dev) * 2 - 1
+            y = torch.empty_like(x)
+            cfg = lib.fft_alloc(n)
+            return (lambda: lib.fft_batch(cfg, x, y, b, n, n, stream=stream)), 16 * n * b
+        if name in ("r2c_1m", "c2r_1m"):
+            n, b = 1 << 20, 256
+            x = torch.rand(b, n, device=dev)
+            X = torch.rand(b, n // 2 + 1, 2, device=dev)
+            nb = (4 * n + 8 * (n // 2 + 1)) * b
+            if name == "r2c_1m":
+                cfg = lib.fftr_alloc(n)
+                return (lambda: lib.fftr_batch(cfg, x, X, b, n, n // 2 + 1, stream=stream)), nb
+            cfg = lib.fftr_alloc(n, True)
+            return (lambda: lib.fftri_batch(cfg, X, x, b, n // 2 + 1, n, stream=stream)), nb
+        if name == "nd_64x96x120":
+            dims = (64, 96, 120)
+            x = torch.rand((16,) + dims + (2,), device=dev) * 2 - 1
+            y = torch.empty_like(x)
+            cfg = lib.fftnd_alloc(dims)
+            return (lambda: lib.fftnd_batch(cfg, x, y, 16, stream=stream)), 16 * 16 * 64 * 96 * 120
+        if name in ("ndr_1024x1000", "ndri_1024x1000"):
+            d2 = (1024, 1000)
+            xr = torch.randn((32,) + d2, device=dev)
+            Xr = torch.randn(32, 1024, 501, 2, device=dev)
+            nb = 32 * (4 * 1024 * 1000 + 8 * 1024 * 501)
+            if name == "ndr_1024x1000":
+                cfg = lib.fftndr_alloc(d2)
+                return (lambda: lib.fftndr_batch(cfg, xr, Xr, 32, stream=stream)), nb
+            cfg = lib.fftndr_alloc(d2, True)
+            return (lambda: lib.fftndri_batch(cfg, Xr, xr, 32, stream=stream)), nb
+        if name == "c2c_1200":
+            x = torch.rand(256, 1200, 2, device=dev) * 2 - 1
+            y = torch.empty_like(x)
+            cfg = lib.fft_alloc(1200)
+            return (lambda: lib.fft_batch(cfg, x, y, 256, 1200, 1200, stream=stream)), 256 * 16 * 1200
+        if name == "q15_512":
+            lq = kissfft_amd.load("int16_t")
+            b = 1 << 20
+            x = torch.randint(-16384, 16384, (b, 512, 2), dtype=torch.int16, device=dev)
+            y = torch.empty_like(x)
+            cfg = lq.fft_alloc(512)
+            return (lambda: lq.fft_batch(cfg, x, y, b, 512, 512, stream=stream)), b * 8 * 512
+        if name == "multi_64":
+            import numpy as np
+            rng = np.random.default_rng(11)
+            M = kissfft_amd.MultiBatch
+            menu = [(M.C2C_FWD, 256), (M.C2C_FWD, 1000), (M.C2C_FWD, 1009), (M.C2C_FWD, 1200), (M.C2C_INV, 3072),
+                    (M.R2C, 4096), (M.C2R, 4000), (M.C2C_FWD, 120)]
+            items, nbytes = [], 0
+            for i in range(64):
+                kind, n = menu[i % len(menu)]
+                cnt = int(np.exp(rng.uniform(0, np.log(512))))
+                nin = (n * cnt * 2) if kind in (0, 1) else (n * cnt if kind == 2 else (n // 2 + 1) * cnt * 2)
+                nout = (n * cnt * 2) if kind in (0, 1) else ((n // 2 + 1) * cnt * 2 if kind == 2 else n * cnt)
+                items.append((kind, n, torch.rand(nin, device=dev) * 2 - 1, torch.empty(nout, device=dev), cnt))
+                nbytes += 4 * (nin + nout)
+            stream.synchronize()
+            mb = M(lib, items)
+            mb.capture(stream)
+            return (lambda: mb.replay(stream)), nbytes
+    raise ValueError(name)
+
+
+PROFILE_WORKLOADS = ("c2c_1m", "c2c_64k", "r2c_1m", "c2r_1m", "nd_64x96x120", "ndr_1024x1000", "ndri_1024x1000",
+                     "c2c_1200", "q15_512", "multi_64")
+
+
+def profile_workload(name, lib, stream, dev) -> int:
+    import torch
+    call, nbytes = _profile_setup(name, lib, stream, dev)
+    call()  # plans, kernel choices, tuning
+    stream.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(3):
+        call()
+    e1.record(stream)
+    e1.synchronize()
+    s = e0.elapsed_time(e1) / 1e3 / 3
+    print(json.dumps({"profile": name, "us_per_call": round(s * 1e6, 1), "alg_bytes": nbytes,
+                      "GBps": round(nbytes / s / 1e9, 1)}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -457,6 +568,9 @@ def main() -> int:
     ap.add_argument("--pmc-q15", action="store_true",
                     help="for rocprofv3: Q15 c2c N=512 over 2^20 transforms, three launches after "
                          "the plan's first call, and exit")
+    ap.add_argument("--profile", choices=sorted(PROFILE_WORKLOADS),
+                    help="for rocprofv3: one workload -- its first call (plans, tuning), then three "
+                         "timed calls on one stream -- and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (batches shard "
@@ -473,6 +587,9 @@ def main() -> int:
     dev = torch.device("cuda:0")
     lib = kissfft_amd.load("float")
     stream = torch.cuda.Stream(device=dev)
+
+    if args.profile:
+        return profile_workload(args.profile, lib, stream, dev)
 
     if args.pmc_long_real:
         n, b = 1 << 20, 256
@@ -536,7 +653,13 @@ def main() -> int:
             lib.fftr_batch(fcfg, x, X, BATCH, n, n // 2 + 1, stream=stream)
             lib.fftri_batch(icfg, X, y, BATCH, n // 2 + 1, n, stream=stream)
 
-    for _ in range(args.warmup):
+    # the first step also builds the plans and times the kernel variants of the four
+    # (direction, length) keys: reported as the first-use cost
+    t_first = time.perf_counter()
+    step()
+    torch.cuda.synchronize()
+    first_step_s = time.perf_counter() - t_first
+    for _ in range(max(0, args.warmup - 1)):
         step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -619,6 +742,7 @@ def main() -> int:
         "other_configs": others,
         "serving": serve,
         "transforms_per_s": round(transforms_per_s, 1),
+        "first_step_ms": round(first_step_s * 1e3, 1),
         "per_kernel": per_kernel,
         "roofline": {
             "bound": "hbm",

@@ -16,8 +16,8 @@
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kUnroll = 4;
 
+template <int kUnroll>
 __global__ void __launch_bounds__(kThreads) k_read(const uint4 *__restrict__ src, size_t nvec, uint4 *sink)
 {
     uint4 acc = make_uint4(0, 0, 0, 0);
@@ -58,6 +58,7 @@ __global__ void __launch_bounds__(kThreads) k_read(const uint4 *__restrict__ src
     }
 }
 
+template <int kUnroll>
 __global__ void __launch_bounds__(kThreads) k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t nvec)
 {
     const size_t stride = (size_t)gridDim.x * kThreads;
@@ -101,13 +102,13 @@ __global__ void __launch_bounds__(kThreads) k_rows(const uint4 *__restrict__ src
     }
 }
 
-int grid_for(size_t nvec)
+int grid_for(size_t nvec, int blocks_per_cu)
 {
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess)
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     const size_t want = (nvec + kThreads - 1) / kThreads;
-    const size_t cap = (size_t)cus * 16;
+    const size_t cap = (size_t)cus * (blocks_per_cu > 0 ? blocks_per_cu : 16);
     return (int)(want < cap ? (want ? want : 1) : cap);
 }
 
@@ -115,25 +116,39 @@ int grid_for(size_t nvec)
 
 extern "C" {
 
-// sink: at least 16 * (16 * CUs) bytes of device memory
-__attribute__((visibility("default"))) int kfa_mb_read(const void *src, size_t bytes, void *sink, void *stream)
+// sink: at least 16 bytes per workgroup (16 * blocks_per_cu * CUs); blocks_per_cu <= 0: 16;
+// unroll: 16-byte loads in flight per thread, 4 or 8
+__attribute__((visibility("default"))) int kfa_mb_read(const void *src, size_t bytes, void *sink, void *stream,
+                                                       int blocks_per_cu, int unroll)
 {
     const size_t nvec = bytes / 16;
-    hipLaunchKernelGGL(k_read, dim3(grid_for(nvec)), dim3(kThreads), 0, (hipStream_t)stream,
-                       (const uint4 *)src, nvec, (uint4 *)sink);
+    const dim3 g(grid_for(nvec, blocks_per_cu));
+    if (unroll == 8)
+        hipLaunchKernelGGL(k_read<8>, g, dim3(kThreads), 0, (hipStream_t)stream, (const uint4 *)src, nvec,
+                           (uint4 *)sink);
+    else
+        hipLaunchKernelGGL(k_read<4>, g, dim3(kThreads), 0, (hipStream_t)stream, (const uint4 *)src, nvec,
+                           (uint4 *)sink);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-__attribute__((visibility("default"))) int kfa_mb_copy(const void *src, void *dst, size_t bytes, void *stream)
+__attribute__((visibility("default"))) int kfa_mb_copy(const void *src, void *dst, size_t bytes, void *stream,
+                                                       int blocks_per_cu, int unroll)
 {
     const size_t nvec = bytes / 16;
-    hipLaunchKernelGGL(k_copy, dim3(grid_for(nvec)), dim3(kThreads), 0, (hipStream_t)stream,
-                       (const uint4 *)src, (uint4 *)dst, nvec);
+    const dim3 g(grid_for(nvec, blocks_per_cu));
+    if (unroll == 8)
+        hipLaunchKernelGGL(k_copy<8>, g, dim3(kThreads), 0, (hipStream_t)stream, (const uint4 *)src, (uint4 *)dst,
+                           nvec);
+    else
+        hipLaunchKernelGGL(k_copy<4>, g, dim3(kThreads), 0, (hipStream_t)stream, (const uint4 *)src, (uint4 *)dst,
+                           nvec);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
 // n must be 4096 or 1024 (one or four 16-byte words per thread); src 16-byte aligned
-__attribute__((visibility("default"))) int kfa_mb_rows(const void *src, void *dst, long long rows, int n, void *stream)
+__attribute__((visibility("default"))) int kfa_mb_rows(const void *src, void *dst, long long rows, int n,
+                                                       void *stream)
 {
     if (rows <= 0 || rows > 0x7fffffffLL || (reinterpret_cast<uintptr_t>(src) & 15) ||
         (reinterpret_cast<uintptr_t>(dst) & 7))

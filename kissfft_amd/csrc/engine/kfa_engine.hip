@@ -221,19 +221,29 @@ size_t scratch_room()
 static std::mutex g_tune_mu;
 static std::map<int, hipStream_t> g_tune_streams;
 
-int tune_stream(hipStream_t *s)
+// The device's tuning stream, made to wait for the work already queued on the caller's
+// stream: candidates are timed after it, not beside it (kernels running alongside would
+// decide the ranking).
+int tune_stream(hipStream_t caller, hipStream_t *s)
 {
     int dev = 0;
     KFA_HIP(hipGetDevice(&dev));
-    std::lock_guard<std::mutex> g(g_tune_mu);
-    auto it = g_tune_streams.find(dev);
-    if (it == g_tune_streams.end()) {
-        hipStream_t t = nullptr;
-        KFA_HIP(hipStreamCreateWithFlags(&t, hipStreamNonBlocking));  // default priority
-        it = g_tune_streams.emplace(dev, t).first;
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        auto it = g_tune_streams.find(dev);
+        if (it == g_tune_streams.end()) {
+            hipStream_t t = nullptr;
+            KFA_HIP(hipStreamCreateWithFlags(&t, hipStreamNonBlocking));  // default priority
+            it = g_tune_streams.emplace(dev, t).first;
+        }
+        *s = it->second;
     }
-    *s = it->second;
-    return 0;
+    hipEvent_t ev = nullptr;
+    KFA_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    const hipError_t e1 = hipEventRecord(ev, caller);
+    const hipError_t e2 = e1 == hipSuccess ? hipStreamWaitEvent(*s, ev, 0) : e1;
+    (void)hipEventDestroy(ev);
+    return e2 == hipSuccess ? 0 : fail(-3, "tuning stream: %s", hipGetErrorString(e2));
 }
 
 bool stream_capturing(hipStream_t s)
@@ -630,10 +640,10 @@ struct TuneBuf {
 // buffers cannot be had the smaller sizes are tried, then the first candidate is taken.
 constexpr long long kTuneValues = 64LL << 20;
 
-static size_t tune_tiles(int n, int mode, const std::vector<std::shared_ptr<TileChoice>> &cand)
+static size_t tune_tiles(int n, int mode, const std::vector<std::shared_ptr<TileChoice>> &cand, hipStream_t s)
 {
     hipStream_t ts;
-    if (tune_stream(&ts))
+    if (tune_stream(s, &ts))
         return 0;
     const long long C = 1024;
     const bool real = mode == TILE_RSPLIT || mode == TILE_RMERGE;
@@ -720,7 +730,7 @@ static int tile_choice(int n, int inverse, int mode, hipStream_t s, std::shared_
     }
     size_t best = 0;
     if (cand.size() > 1 && env_flag("KISSFFT_AMD_TUNE", true) && !stream_capturing(s))
-        best = tune_tiles(n, mode, cand);
+        best = tune_tiles(n, mode, cand, s);
     std::shared_ptr<const TileChoice> tc = cand.empty() ? std::make_shared<const TileChoice>() : cand[best];
     if (tc->v && env_flag("KISSFFT_AMD_VERBOSE", false))
         fprintf(stderr, "[kissfft_amd] tile n=%d inverse=%d mode=%d: %s\n", n, inverse, mode, tc->v->name);
@@ -1017,7 +1027,7 @@ static int plane_choice(int A, int B, int inverse, hipStream_t s, std::shared_pt
     }
     size_t best = 0;
     hipStream_t ts;
-    if (cand.size() > 1 && env_flag("KISSFFT_AMD_TUNE", true) && !stream_capturing(s) && tune_stream(&ts) == 0) {
+    if (cand.size() > 1 && env_flag("KISSFFT_AMD_TUNE", true) && !stream_capturing(s) && tune_stream(s, &ts) == 0) {
         for (long long values = kTuneValues; values >= (kTuneValues >> 4); values >>= 2) {
             const long long planes = std::max(1LL, values / ((long long)A * B));
             const size_t bytes = 8 * (size_t)A * B * (size_t)planes;
@@ -1207,7 +1217,7 @@ int choose_fast(DevPlan &P, int kind, hipStream_t s, const FastChoice **out, lon
                 from_table = true;
             }
     hipStream_t ts;
-    if (tune && !from_table && !stream_capturing(s) && tune_stream(&ts) == 0) {
+    if (tune && !from_table && !stream_capturing(s) && tune_stream(s, &ts) == 0) {
         long long lines = tune_lines;
         for (; lines >= 64; lines /= 4) {
             const size_t bytes = (size_t)per_line * (size_t)lines;

@@ -211,6 +211,10 @@ const std::vector<FastVariant> &fast_registry()
         add_wave<16, 4, false, 2, 4, 4, 4, 4>(*v, "q15w_c2c512_t16x4");
         add_wave<64, 4, false, 2, 4, 4, 4, 4>(*v, "q15w_c2c512_t64x4");
         add_wave<32, 4, false, 2, 4, 4, 4, 4>(*v, "q15w_c2c512_t32x4");
+        add_wave<16, 16, false, 4, 4, 4, 4>(*v, "q15w_c2c256_t16x16");
+        add_wave<32, 8, false, 4, 4, 4, 4, 4>(*v, "q15w_c2c1024_t32x8");
+        add_wave<16, 16, false, 4, 4, 4, 4, 4>(*v, "q15w_c2c1024_t16x16");
+        add_wave<64, 4, false, 2, 4, 4, 4, 4, 4>(*v, "q15w_c2c2048_t64x4");
         // ---- Q15 (reference schedules), packed-word butterflies ----
         add_variant<Q15P, FK_C2C, 32, 8, false, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c512_t32x8_sb");
         add_variant<Q15P, FK_C2C, 16, 16, false, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c512_t16x16_sb");

@@ -198,9 +198,10 @@ int blue_prepare(DevPlan &P, bool *applies, std::shared_ptr<const BlueChoice> *k
 struct BlueArgs;
 int blue_launch(DevPlan &P, BlueArgs &b, hipStream_t s);
 
-// Tuning (first-use timing of variants) runs on a private stream of the device, never on
-// a caller's stream, and not at all while the caller's stream is being captured.
-int tune_stream(hipStream_t *s);
+// Tuning (first-use timing of variants) runs on a private stream of the device, after the
+// work queued on the caller's stream, never on that stream itself, and not at all while the
+// caller's stream is being captured.
+int tune_stream(hipStream_t caller, hipStream_t *s);
 bool stream_capturing(hipStream_t s);
 // Plan tables are uploaded with synchronous copies; while this object lives the calling
 // thread's capture mode is relaxed, so such uploads do not invalidate a capture in progress.

@@ -10,6 +10,12 @@ for p in (ROOT, os.path.join(ROOT, "oracle"), os.path.join(ROOT, "tests"), os.pa
         sys.path.insert(0, p)
 
 
+# Tests race first-use tuning between threads and force variants by name: their timings
+# must not reach the persisted table of tuned choices that later processes (a benchmark)
+# reuse.  The tests of the table point it at a temporary file of their own.
+os.environ["KISSFFT_AMD_TUNE_TABLE"] = "0"
+
+
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP engine)")
 

@@ -28,6 +28,7 @@ def say(*a):
 @pytest.fixture
 def rtc_env(monkeypatch, tmp_path):
     monkeypatch.setenv("KISSFFT_AMD_CACHE", str(tmp_path))
+    monkeypatch.setenv("KISSFFT_AMD_TUNE_TABLE", str(tmp_path / "tuned.txt"))
     monkeypatch.setenv("KISSFFT_AMD_VERBOSE", "1")
     return tmp_path
 
