@@ -161,29 +161,34 @@ struct Ops<FX<BITS>> {
 // w[r-1] is the stage twiddle for input r (table entry r*u*fstride); ya/yb are the
 // fixed rotations table[fstride*m] and table[2*fstride*m] (radix 3 uses ya only).
 // Results overwrite v in output order.
+//
+// ONE: every twiddle of the butterfly is table entry 0, (SAMP_MAX, 0) in fixed point.
+// C_MUL by it returns its operand for every component in (-2^(B-1), 2^(B-1)] (B the
+// fraction bits), and C_FIXDIV by p >= 2 always lands there, so the product is left out
+// with the same bits.  Only fixed-point stages pass it.
 // ------------------------------------------------------------------------------------
 
 // kiss_fft.c:15-36
-template <class A>
+template <class A, bool ONE = false>
 __device__ __forceinline__ void bfly2(Cx<A> *v, const Cx<A> *w)
 {
     typedef Ops<A> O;
     Cx<A> a = O::fixdiv(v[0], 2), b = O::fixdiv(v[1], 2);
-    Cx<A> t = O::cmul(b, w[0]);
+    Cx<A> t = ONE ? b : O::cmul(b, w[0]);
     v[1] = O::csub(a, t);
     v[0] = O::cadd(a, t);
 }
 
 // kiss_fft.c:38-84
-template <class A>
+template <class A, bool ONE = false>
 __device__ __forceinline__ void bfly4(Cx<A> *v, const Cx<A> *w, bool inverse)
 {
     typedef Ops<A> O;
     Cx<A> x0 = O::fixdiv(v[0], 4), x1 = O::fixdiv(v[1], 4);
     Cx<A> x2 = O::fixdiv(v[2], 4), x3 = O::fixdiv(v[3], 4);
-    Cx<A> s0 = O::cmul(x1, w[0]);
-    Cx<A> s1 = O::cmul(x2, w[1]);
-    Cx<A> s2 = O::cmul(x3, w[2]);
+    Cx<A> s0 = ONE ? x1 : O::cmul(x1, w[0]);
+    Cx<A> s1 = ONE ? x2 : O::cmul(x2, w[1]);
+    Cx<A> s2 = ONE ? x3 : O::cmul(x3, w[2]);
     Cx<A> s5 = O::csub(x0, s1);
     x0 = O::cadd(x0, s1);
     Cx<A> s3 = O::cadd(s0, s2);
@@ -200,13 +205,13 @@ __device__ __forceinline__ void bfly4(Cx<A> *v, const Cx<A> *w, bool inverse)
 }
 
 // kiss_fft.c:86-128
-template <class A>
+template <class A, bool ONE = false>
 __device__ __forceinline__ void bfly3(Cx<A> *v, const Cx<A> *w, Cx<A> epi3)
 {
     typedef Ops<A> O;
     Cx<A> x0 = O::fixdiv(v[0], 3), x1 = O::fixdiv(v[1], 3), x2 = O::fixdiv(v[2], 3);
-    Cx<A> s1 = O::cmul(x1, w[0]);
-    Cx<A> s2 = O::cmul(x2, w[1]);
+    Cx<A> s1 = ONE ? x1 : O::cmul(x1, w[0]);
+    Cx<A> s2 = ONE ? x2 : O::cmul(x2, w[1]);
     Cx<A> s3 = O::cadd(s1, s2);
     Cx<A> s0 = O::csub(s1, s2);
     Cx<A> o1{O::sub(x0.r, O::half(s3.r)), O::sub(x0.i, O::half(s3.i))};
@@ -220,16 +225,20 @@ __device__ __forceinline__ void bfly3(Cx<A> *v, const Cx<A> *w, Cx<A> epi3)
 }
 
 // kiss_fft.c:130-189
-template <class A>
+template <class A, bool ONE = false>
 __device__ __forceinline__ void bfly5(Cx<A> *v, const Cx<A> *w, Cx<A> ya, Cx<A> yb)
 {
     typedef Ops<A> O;
     Cx<A> x0 = O::fixdiv(v[0], 5);
     const Cx<A> c0 = x0;
-    Cx<A> c1 = O::cmul(O::fixdiv(v[1], 5), w[0]);
-    Cx<A> c2 = O::cmul(O::fixdiv(v[2], 5), w[1]);
-    Cx<A> c3 = O::cmul(O::fixdiv(v[3], 5), w[2]);
-    Cx<A> c4 = O::cmul(O::fixdiv(v[4], 5), w[3]);
+    Cx<A> c1 = O::fixdiv(v[1], 5), c2 = O::fixdiv(v[2], 5);
+    Cx<A> c3 = O::fixdiv(v[3], 5), c4 = O::fixdiv(v[4], 5);
+    if constexpr (!ONE) {
+        c1 = O::cmul(c1, w[0]);
+        c2 = O::cmul(c2, w[1]);
+        c3 = O::cmul(c3, w[2]);
+        c4 = O::cmul(c4, w[3]);
+    }
     Cx<A> c7 = O::cadd(c1, c4), c10 = O::csub(c1, c4);
     Cx<A> c8 = O::cadd(c2, c3), c9 = O::csub(c2, c3);
 

@@ -436,6 +436,8 @@ struct FastStage {
 
     __device__ __forceinline__ static void compute(C (&v)[J][p], const St *__restrict__ stw, int t)
     {
+        // m == 1: every butterfly's twiddles are table entry 0 (see ONE in kfa_arith.hpp)
+        constexpr bool ONE = A::fixed && m == 1;
 #pragma unroll
         for (int j = 0; j < J; ++j) {
             if (!live(t, j))
@@ -447,45 +449,49 @@ struct FastStage {
                 // stage table of pre-arranged twiddle word pairs (see kfa_q15p.hpp)
                 const uint2 *tab = reinterpret_cast<const uint2 *>(stw);
                 const uint2 *trow = tab + TWO + u * PR;
-                uint2 w[p > 1 ? p - 1 : 1];
+                uint2 w[p > 1 ? p - 1 : 1] = {};
+                if constexpr (!ONE) {
 #pragma unroll
-                for (int r = 1; r < p; ++r)
-                    w[r - 1] = trow[r - 1];
+                    for (int r = 1; r < p; ++r)
+                        w[r - 1] = trow[r - 1];
+                }
                 unsigned x[p];
 #pragma unroll
                 for (int r = 0; r < p; ++r)
                     x[r] = v[j][r].w;
                 if constexpr (p == 2) {
-                    q15p::bfly2(x, w);
+                    q15p::bfly2<ONE>(x, w);
                 } else if constexpr (p == 4) {
-                    q15p::bfly4<INV>(x, w);
+                    q15p::bfly4<INV, ONE>(x, w);
                 } else if constexpr (p == 3) {
                     const uint2 ya = tab[TWO + m * PR];
-                    q15p::bfly3(x, w, q15p::scal(q15p::lo16(ya.y)));
+                    q15p::bfly3<ONE>(x, w, q15p::scal(q15p::lo16(ya.y)));
                 } else {
                     const uint2 ya = tab[TWO + m * PR], yb = tab[TWO + m * PR + 1];
                     q15p::Y5 y{q15p::scal(q15p::lo16(ya.x)), q15p::scal(q15p::lo16(ya.y)),
                                q15p::scal(q15p::lo16(yb.x)), q15p::scal(q15p::lo16(yb.y))};
-                    q15p::bfly5(x, w, y);
+                    q15p::bfly5<ONE>(x, w, y);
                 }
 #pragma unroll
                 for (int r = 0; r < p; ++r)
                     v[j][r].w = x[r];
             } else if constexpr (A::fixed) {
-                C w[p > 1 ? p - 1 : 1];
+                C w[p > 1 ? p - 1 : 1] = {};
+                if constexpr (!ONE) {
 #pragma unroll
-                for (int r = 1; r < p; ++r)
-                    w[r - 1] = O::load(row[r - 1]);
+                    for (int r = 1; r < p; ++r)
+                        w[r - 1] = O::load(row[r - 1]);
+                }
                 const C ya = O::load(stw[TWO + m * PR]);
                 const C yb = O::load(stw[TWO + m * PR + 1]);
                 if constexpr (p == 2)
-                    bfly2<A>(v[j], w);
+                    bfly2<A, ONE>(v[j], w);
                 else if constexpr (p == 3)
-                    bfly3<A>(v[j], w, ya);
+                    bfly3<A, ONE>(v[j], w, ya);
                 else if constexpr (p == 4)
-                    bfly4<A>(v[j], w, INV);
+                    bfly4<A, ONE>(v[j], w, INV);
                 else
-                    bfly5<A>(v[j], w, ya, yb);
+                    bfly5<A, ONE>(v[j], w, ya, yb);
                 (void)yb;
             } else {
                 if constexpr (m > 1) {

@@ -112,6 +112,42 @@ static void add_wave(std::vector<FastVariant> &v, const char *name)
     }
 }
 
+template <bool INV, int LPB, class RS, int WPE>
+static int launch_xpose(const FastArgs &a, hipStream_t s)
+{
+    const long long blocks = (a.nlines + LPB - 1) / LPB;
+    if (blocks <= 0)
+        return 0;
+    hipLaunchKernelGGL((k_q15x<INV, LPB, RS, WPE>), dim3((unsigned)blocks), dim3(16 * LPB), 0, s, a);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+// Q15 complex lines of 16 lanes with one LDS transpose in place of the lane swaps
+// (k_q15x, kfa_q15w.hpp); radix 2 / 4 schedules of 256 or 512 points; WPE: wavefronts per
+// SIMD the registers must allow (1: the compiler's choice)
+template <int LPB, int WPE, int... R>
+static void add_xpose(std::vector<FastVariant> &v, const char *name)
+{
+    typedef Radices<TwPad<Q15P>::G, R...> RS;
+    static_assert(16 * LPB <= 1024, "workgroup too large");
+    for (int inv = 0; inv < 2; ++inv) {
+        FastVariant f;
+        f.type = KFA_Q15;
+        f.kind = FK_C2C;
+        f.n = RS::N();
+        f.inverse = inv;
+        f.T = 16;
+        f.LPB = LPB;
+        f.radices = {R...};
+        f.tw_pad = TwPad<Q15P>::G;
+        f.tw_fmt = TwPad<Q15P>::fmt;
+        f.tw_len = RS::tw_len();
+        f.name = name;
+        f.launch = inv ? &launch_xpose<true, LPB, RS, WPE> : &launch_xpose<false, LPB, RS, WPE>;
+        v.push_back(f);
+    }
+}
+
 // forward and inverse real variants of one schedule, split / merge in a separate LDS pass
 template <class A, int T, int LPB, bool DB, int... R>
 static void add_real(std::vector<FastVariant> &v, int type, const char *name)
@@ -203,6 +239,11 @@ const std::vector<FastVariant> &fast_registry()
         add_variant<F32, FK_C2C, 192, 1, false, false, 3, 16, 8, 8>(*v, KFA_F32, "c2c3072_31688_t192_sb");
         add_variant<F32, FK_C2C, 256, 1, false, false, 16, 16, 16>(*v, KFA_F32, "c2c4096_161616_t256_sb");
         // ---- Q15 (reference schedules), wavefront-resident lines: lane-swap exchanges ----
+        add_xpose<8, 1, 2, 4, 4, 4, 4>(*v, "q15x_c2c512_t16x8");
+        add_xpose<16, 1, 2, 4, 4, 4, 4>(*v, "q15x_c2c512_t16x16");
+        add_xpose<8, 6, 2, 4, 4, 4, 4>(*v, "q15x_c2c512_t16x8_o6");
+        add_xpose<4, 6, 2, 4, 4, 4, 4>(*v, "q15x_c2c512_t16x4_o6");
+        add_xpose<16, 1, 4, 4, 4, 4>(*v, "q15x_c2c256_t16x16");
         add_wave<16, 16, true, 2, 4, 4, 4, 4>(*v, "q15w_c2c512_t16x16_asm");
         add_wave<32, 8, true, 2, 4, 4, 4, 4>(*v, "q15w_c2c512_t32x8_asm");
         add_wave<16, 16, false, 2, 4, 4, 4, 4>(*v, "q15w_c2c512_t16x16");

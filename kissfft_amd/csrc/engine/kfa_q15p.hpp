@@ -96,8 +96,25 @@ __device__ __forceinline__ int dot(u32 a, u32 b)
 #endif
 }
 
+// The same product with the rounding constant in a vector register and b in a scalar or a
+// vector register: a twiddle read with a scalar load (a row that is the same on every
+// lane) then feeds the product directly instead of through a register copy (one scalar
+// operand per instruction).
+__device__ __forceinline__ int dot_tw(u32 a, u32 b)
+{
+#if KFA_Q15_DOT_ASM
+    int rc = 1 << 14;
+    asm("" : "+v"(rc));
+    int r;
+    asm("v_dot2_i32_i16 %0, %1, %2, %3" : "=v"(r) : "v"(a), "r"(b), "v"(rc));
+    return r;
+#else
+    return __builtin_amdgcn_sdot2(as_v(a), as_v(b), 1 << 14, false);
+#endif
+}
+
 // complex product with a pre-arranged twiddle pair (w.r, -w.i), (w.i, w.r)
-__device__ __forceinline__ u32 cmul(u32 a, u32 wr, u32 wi) { return round_pack(dot(a, wr), dot(a, wi)); }
+__device__ __forceinline__ u32 cmul(u32 a, u32 wr, u32 wi) { return round_pack(dot_tw(a, wr), dot_tw(a, wi)); }
 
 // both components times the scalar s: pass (s, 0) and (0, s)
 __device__ __forceinline__ u32 scale(u32 a, u32 s_lo, u32 s_hi) { return round_pack(dot(a, s_lo), dot(a, s_hi)); }
@@ -108,9 +125,19 @@ struct Scal {
 __device__ __forceinline__ Scal scal(int s) { return Scal{(u32)s & 0xffffu, (u32)s << 16}; }
 __device__ __forceinline__ u32 scale(u32 a, Scal s) { return scale(a, s.lo, s.hi); }
 
-// -i * z = (z.i, -z.r) and +i * z = (-z.i, z.r)
-__device__ __forceinline__ u32 mul_mi(u32 z) { return mul_lo(swap(z), 0xffff0001u); }
-__device__ __forceinline__ u32 mul_pi(u32 z) { return mul_lo(swap(z), 0x0001ffffu); }
+// -i * z = (z.i, -z.r) and +i * z = (-z.i, z.r): one packed multiply by (1, -1) / (-1, 1)
+// whose source halves are exchanged by its operand selection (written as a vector shuffle,
+// which the compiler folds into op_sel of the multiply or of the add that uses it)
+__device__ __forceinline__ u32 mul_mi(u32 z)
+{
+    const v2s zz = as_v(z);
+    return as_u(__builtin_shufflevector(zz, zz, 1, 0) * (v2s){1, -1});
+}
+__device__ __forceinline__ u32 mul_pi(u32 z)
+{
+    const v2s zz = as_v(z);
+    return as_u(__builtin_shufflevector(zz, zz, 1, 0) * (v2s){-1, 1});
+}
 
 __device__ __forceinline__ int lo16(u32 w) { return (int)(short)(w & 0xffffu); }
 
@@ -169,8 +196,8 @@ __device__ __forceinline__ void cmul_n(u32 (&s)[K], const u32 *a, const uint2 *t
     int re[K], im[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        re[k] = dot(a[k], tw[k].x);
-        im[k] = dot(a[k], tw[k].y);
+        re[k] = dot_tw(a[k], tw[k].x);
+        im[k] = dot_tw(a[k], tw[k].y);
     }
 #if KFA_Q15_SDWA_PACK
     // round_pack's two shifts, each kind for all K values in a row
@@ -189,24 +216,35 @@ __device__ __forceinline__ void cmul_n(u32 (&s)[K], const u32 *a, const uint2 *t
 #endif
 }
 
+// ONE: every twiddle of the butterfly is table entry 0, (32767, 0).  C_MUL by it returns
+// its operand for every component in (-2^14, 2^14], and C_FIXDIV by p >= 2 always lands
+// there (|sround(c * floor(32767 / p))| <= 16383), so the product is left out: same bits.
+
 // kiss_fft.c:15-36
+template <bool ONE = false>
 __device__ __forceinline__ void bfly2(u32 *v, const uint2 *tw)
 {
     u32 x[2];
     fixdiv_n<2, 2>(x, v);
     const u32 a = x[0];
-    u32 t = cmul(x[1], tw[0].x, tw[0].y);
+    u32 t = ONE ? x[1] : cmul(x[1], tw[0].x, tw[0].y);
     v[1] = sub(a, t);
     v[0] = add(a, t);
 }
 
 // kiss_fft.c:38-84
-template <bool INV>
+template <bool INV, bool ONE = false>
 __device__ __forceinline__ void bfly4(u32 *v, const uint2 *tw)
 {
     u32 x[4], s[3];
     fixdiv_n<4, 4>(x, v);
-    cmul_n<3>(s, x + 1, tw);
+    if constexpr (ONE) {
+        s[0] = x[1];
+        s[1] = x[2];
+        s[2] = x[3];
+    } else {
+        cmul_n<3>(s, x + 1, tw);
+    }
     u32 x0 = x[0];
     u32 s0 = s[0], s1 = s[1], s2 = s[2];
     u32 s5 = sub(x0, s1);
@@ -215,35 +253,32 @@ __device__ __forceinline__ void bfly4(u32 *v, const uint2 *tw)
     u32 s4 = sub(s0, s2);
     v[2] = sub(x0, s3);
     v[0] = add(x0, s3);
-    u32 s4s = swap(s4);
-    u32 A = add(s5, s4s);  // (s5.r + s4.i, s5.i + s4.r)
-    u32 B = sub(s5, s4s);  // (s5.r - s4.i, s5.i - s4.r)
+    const u32 u = mul_mi(s4);  // (s4.i, -s4.r)
     if (INV) {
-        v[1] = lohi(B, A);
-        v[3] = lohi(A, B);
+        v[1] = sub(s5, u);  // (s5.r - s4.i, s5.i + s4.r)
+        v[3] = add(s5, u);  // (s5.r + s4.i, s5.i - s4.r)
     } else {
-        v[1] = lohi(A, B);
-        v[3] = lohi(B, A);
+        v[1] = add(s5, u);
+        v[3] = sub(s5, u);
     }
 }
 
 // kiss_fft.c:86-128; e = (epi3.i, 0), (0, epi3.i)
+template <bool ONE = false>
 __device__ __forceinline__ void bfly3(u32 *v, const uint2 *tw, Scal e)
 {
     u32 x0 = fixdiv<3>(v[0]), x1 = fixdiv<3>(v[1]), x2 = fixdiv<3>(v[2]);
-    u32 s1 = cmul(x1, tw[0].x, tw[0].y);
-    u32 s2 = cmul(x2, tw[1].x, tw[1].y);
+    u32 s1 = ONE ? x1 : cmul(x1, tw[0].x, tw[0].y);
+    u32 s2 = ONE ? x2 : cmul(x2, tw[1].x, tw[1].y);
     u32 s3 = add(s1, s2);
     u32 s0 = sub(s1, s2);
     u32 o1 = sub(x0, half(s3));
     s0 = scale(s0, e);
     x0 = add(x0, s3);
-    u32 s0s = swap(s0);
-    u32 A = add(o1, s0s);  // (o1.r + s0.i, o1.i + s0.r)
-    u32 B = sub(o1, s0s);  // (o1.r - s0.i, o1.i - s0.r)
+    const u32 u = mul_mi(s0);  // (s0.i, -s0.r)
     v[0] = x0;
-    v[1] = lohi(B, A);
-    v[2] = lohi(A, B);
+    v[1] = sub(o1, u);  // (o1.r - s0.i, o1.i + s0.r)
+    v[2] = add(o1, u);  // (o1.r + s0.i, o1.i - s0.r)
 }
 
 // kiss_fft.c:130-189
@@ -251,13 +286,17 @@ struct Y5 {
     Scal yar, yai, ybr, ybi;
 };
 
+template <bool ONE = false>
 __device__ __forceinline__ void bfly5(u32 *v, const uint2 *tw, const Y5 &y)
 {
     const u32 x0 = fixdiv<5>(v[0]);
-    u32 c1 = cmul(fixdiv<5>(v[1]), tw[0].x, tw[0].y);
-    u32 c2 = cmul(fixdiv<5>(v[2]), tw[1].x, tw[1].y);
-    u32 c3 = cmul(fixdiv<5>(v[3]), tw[2].x, tw[2].y);
-    u32 c4 = cmul(fixdiv<5>(v[4]), tw[3].x, tw[3].y);
+    u32 c1 = fixdiv<5>(v[1]), c2 = fixdiv<5>(v[2]), c3 = fixdiv<5>(v[3]), c4 = fixdiv<5>(v[4]);
+    if constexpr (!ONE) {
+        c1 = cmul(c1, tw[0].x, tw[0].y);
+        c2 = cmul(c2, tw[1].x, tw[1].y);
+        c3 = cmul(c3, tw[2].x, tw[2].y);
+        c4 = cmul(c4, tw[3].x, tw[3].y);
+    }
     u32 c7 = add(c1, c4), c10 = sub(c1, c4);
     u32 c8 = add(c2, c3), c9 = sub(c2, c3);
     u32 c5 = add(add(x0, scale(c7, y.yar)), scale(c8, y.ybr));

@@ -25,7 +25,8 @@ from oracle import Oracle
 pytestmark = pytest.mark.gpu
 
 C2C_FWD, C2C_INV, R2C, C2R = 0, 1, 2, 3
-Q15_512_VARIANTS = ["q15w_c2c512_t16x16_asm", "q15w_c2c512_t32x8_asm", "q15w_c2c512_t32x8", "q15w_c2c512_t16x16", "q15w_c2c512_t64x4", "q15w_c2c512_t32x4",
+Q15_512_VARIANTS = ["q15x_c2c512_t16x8", "q15x_c2c512_t16x16", "q15x_c2c512_t16x8_o6",
+                    "q15x_c2c512_t16x4_o6", "q15w_c2c512_t16x16_asm", "q15w_c2c512_t32x8_asm", "q15w_c2c512_t32x8", "q15w_c2c512_t16x16", "q15w_c2c512_t64x4", "q15w_c2c512_t32x4",
                     "q15p_c2c512_t32x8_sb", "q15p_c2c512_t16x16_sb", "q15p_c2c512_t32x8_db",
                     "q15p_c2c512_t64x4_sb", "q15p_c2c512_t64x4_db", "q15p_c2c512_t128x2_sb",
                     "q15c2c512_t128x2_db", "q15c2c512_t128x2_sb", "q15c2c512_t64x4_sb",
@@ -53,7 +54,7 @@ def _first_mismatch(got, want):
     return int(bad.sum()), (int(idx[0, 0]) if idx.numel() else -1)
 
 
-def test_q15_512_full_batch_bit_exact(q15_full, golden_q15, monkeypatch):
+def test_q15_512_full_batch_bit_exact(q15_full, golden_q15, monkeypatch, capfd):
     import torch
 
     x, want = q15_full
@@ -74,8 +75,12 @@ def test_q15_512_full_batch_bit_exact(q15_full, golden_q15, monkeypatch):
             monkeypatch.setenv("KISSFFT_AMD_VARIANT", name)
         say("case q15 512 x 2^20 variant", name or "tuned")
         yd.fill_(0x5A5A)
+        capfd.readouterr()
         lib.fft_batch(lib.fft_alloc(512), xd, yd, B, 512, 512)
         torch.cuda.synchronize()
+        err = capfd.readouterr().err
+        if name is not None:
+            assert f"fast variant {name}\n" in err, f"variant {name} was not the one launched"
         nbad, first = _first_mismatch(yd, wd)
         assert nbad == 0, f"variant {name or 'tuned'}: {nbad} of {B} transforms differ, first {first}"
     lib.L.kiss_fft_cleanup()

@@ -27,11 +27,11 @@ def registry():
     src = open(REG).read()
     out = []
     fk = {"FK_C2C": "c2c", "FK_R2C": "r2c", "FK_C2R": "c2r"}
-    for m in re.finditer(r"add_(real|variant|fused|wave)<([^>]*)>\(\*v,\s*(?:KFA_\w+,\s*)?\"(\w+)\"\)", src):
+    for m in re.finditer(r"add_(real|variant|fused|wave|xpose)<([^>]*)>\(\*v,\s*(?:KFA_\w+,\s*)?\"(\w+)\"\)", src):
         fn, targs, name = m.groups()
         a = [x.strip() for x in targs.split(",")]
-        if fn == "wave":  # <T, LPB, ASM, radices...>: Q15 complex, wavefront-resident lines
-            n = int(np.prod([int(r) for r in a[3:]]))
+        if fn in ("wave", "xpose"):  # <T, LPB, ASM, radices...> / <LPB, WPE, radices...>: Q15 complex
+            n = int(np.prod([int(r) for r in a[3 if fn == "wave" else 2:]]))
             out.append(("Q15P", "c2c", n, name))
             continue
         if fn == "real":  # <A, T, LPB, DB, radices...>
