@@ -1039,7 +1039,7 @@ static int plane_choice(int A, int B, int inverse, hipStream_t s, std::shared_pt
             char what[64];
             snprintf(what, sizeof what, "plane %dx%d", A, B);
             best = time_candidates(cand.size(), ts, [&](size_t i) {
-                PlaneArgs pa{bin.p, bout.p, cand[i]->stw_a, cand[i]->stw_b, planes};
+                const PlaneArgs pa = plane_args(bin.p, bout.p, cand[i]->stw_a, cand[i]->stw_b, planes, A, B);
                 return cand[i]->v->launch(pa, ts);
             }, what);
             break;
@@ -2232,26 +2232,50 @@ int run_nd(const std::vector<std::shared_ptr<DevPlan>> &P, const int *dims, cons
     long long outer = howmany, inner = total;
     const void *src = in;
     int rc = 0;
-    int last = ndims;  // axes [0, last) run one pass each
+    int paired = -1;  // float: the axis done together with the last one in a plane pass
+    std::shared_ptr<const PlaneChoice> pc;
     if (ndims >= 2 && P[ndims - 1]->type == KFA_F32) {
-        // the last two axes in one pass when a plane of them fits in LDS
-        std::shared_ptr<const PlaneChoice> pc;
-        if ((rc = plane_choice(dims[ndims - 2], dims[ndims - 1], P[ndims - 1]->inverse, s, pc)))
-            return rc;
-        if (pc->v) {
-            PlaneArgs pa{in, out, pc->stw_a, pc->stw_b, howmany * (total / ((long long)dims[ndims - 2] * dims[ndims - 1]))};
+        // the last axis and one other in one pass when a plane (or slab) of them fits in
+        // LDS: the axis before the last first (contiguous planes), then the outer axes from
+        // the first (float results do not depend on the order of the axes beyond rounding)
+        const int nb = dims[ndims - 1];
+        std::vector<int> order{ndims - 2};
+        for (int k = 0; k < ndims - 2; ++k)
+            order.push_back(k);
+        for (int k : order) {
+            long long rs = 1;  // in-plane offsets are 32-bit
+            for (int j = k + 1; j < ndims; ++j)
+                rs *= dims[j];
+            if ((long long)(dims[k] - 1) * rs + nb >= (1LL << 31))
+                continue;
+            if ((rc = plane_choice(dims[k], nb, P[ndims - 1]->inverse, s, pc)))
+                return rc;
+            if (pc->v && (k == ndims - 2 || pc->v->slab)) {
+                paired = k;
+                break;
+            }
+        }
+        if (paired >= 0) {
+            long long rs = 1, mid = 1;  // row stride; planes per value of the axes before `paired`
+            for (int k = paired + 1; k < ndims; ++k)
+                rs *= dims[k];
+            for (int k = paired + 1; k < ndims - 1; ++k)
+                mid *= dims[k];
+            const long long planes = howmany * (total / ((long long)dims[paired] * nb));
+            const PlaneArgs pa{in, out, pc->stw_a, pc->stw_b, planes, (int)rs, mid, (long long)dims[paired] * rs, nb};
             ++g_launches;
             if (pc->v->launch(pa, s))
                 return fail(-3, "plane kernel launch failed");
             src = out;
-            last = ndims - 2;
         }
     }
-    for (int k = 0; k < last && rc == 0; ++k) {
+    for (int k = 0; k < ndims && rc == 0; ++k) {
         inner /= dims[k];
-        rc = run_axis(*P[k], src, out, outer, inner, s);
+        if (paired < 0 || (k != paired && k != ndims - 1)) {
+            rc = run_axis(*P[k], src, out, outer, inner, s);
+            src = out;
+        }
         outer *= dims[k];
-        src = out;
     }
     return rc;
 }

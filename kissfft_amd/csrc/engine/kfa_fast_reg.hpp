@@ -116,6 +116,7 @@ struct PlaneVariant {
     std::vector<int> rad_a, rad_b;
     int tw_pad;
     size_t lds;
+    bool slab;    // also runs slabs of two non-adjacent axes (PlaneArgs rs / nsub / po / pi)
     const char *name;
     int (*launch)(const PlaneArgs &, hipStream_t);
 };

@@ -6,6 +6,13 @@
 // one store: rows of B points, then columns of A points, all in LDS.  An array of d axes
 // then costs d-1 HBM passes instead of d (64 x 64 x 64: 2 instead of 3).
 //
+// A "plane" may also be a slab of two non-adjacent axes: columns along an outer axis k,
+// rows along the last (contiguous) axis, the axes between them fixed per plane.  Plane p
+// then starts at (p / nsub) * po + (p % nsub) * pi and its rows are rs values apart (the
+// last two axes: nsub = 1, po = A * B, rs = B).  Every row is still a contiguous run of B
+// values, so loads and stores stay coalesced; a 3-axis array then costs two passes with a
+// slab of axes 0 and 2 when the plane of axes 1 and 2 does not fit in LDS.
+//
 // Layout in LDS: row r at r*RSX (RSX = B+1 when B is even, so that a column's elements
 // fall in different banks).  Row stages use TB threads per row and unit element stride;
 // column stages TA threads per column and element stride RSX.  Both use the fast kernels'
@@ -23,7 +30,19 @@ struct PlaneArgs {
     const void *stw_a;  // stage table of the column schedule (length A)
     const void *stw_b;  // stage table of the row schedule (length B)
     long long planes;
+    int rs;             // values between the starts of two rows of a plane ((A - 1) * rs + B
+                        // must stay below 2^31)
+    long long nsub;     // plane p starts at (p / nsub) * po + (p % nsub) * pi
+    long long po;
+    long long pi;
 };
+
+// the arguments of contiguous A x B planes
+inline PlaneArgs plane_args(const void *in, void *out, const void *stw_a, const void *stw_b, long long planes,
+                            long long A, long long B)
+{
+    return PlaneArgs{in, out, stw_a, stw_b, planes, (int)B, 1, A * B, 0};
+}
 
 template <bool INV, int T, class RS, int ES, int S>
 __device__ __forceinline__ void plane_stage(F32::store2 *lb, const F32::store2 *stw, int t, bool active)
@@ -52,7 +71,8 @@ __device__ __forceinline__ void plane_stages(std::integer_sequence<int, S...>, F
 // they are in flight while the current plane is transformed; the stage tables are copied
 // to LDS once, so that the stages issue no global loads (a global load's wait would also
 // wait for the prefetch).
-template <bool INV, int TA, int TB, int NTH, class RA, class RB>
+// SLAB: planes addressed through rs / nsub / po / pi (else contiguous A x B planes, p * A * B)
+template <bool INV, int TA, int TB, int NTH, class RA, class RB, bool SLAB>
 __global__ void __launch_bounds__(NTH) k_plane(PlaneArgs a)
 {
     typedef F32::store2 St;
@@ -77,13 +97,16 @@ __global__ void __launch_bounds__(NTH) k_plane(PlaneArgs a)
 
     long long pl = blockIdx.x;
     St v[IT];
+    const int rs = SLAB ? a.rs : B;
+    auto base = [&](long long p) { return SLAB ? (p / a.nsub) * a.po + (p % a.nsub) * a.pi : p * E; };
+    auto at = [&](int e) { return SLAB ? (e / B) * rs + e % B : e; };
     auto load = [&](long long p) {
-        const St *src = in + p * E;
+        const St *src = in + base(p);
 #pragma unroll
         for (int k = 0; k < IT; ++k) {
             const int e = tid + k * NTH;
             if (E % NTH == 0 || e < E)
-                v[k] = src[e];
+                v[k] = src[at(e)];
         }
     };
     if (pl < a.planes)
@@ -109,7 +132,7 @@ __global__ void __launch_bounds__(NTH) k_plane(PlaneArgs a)
                                            twa, tid - line * TA, line < B);
         }
         {
-            St *dst = out + pl * E;
+            St *dst = out + base(pl);
             St w[IT];
 #pragma unroll
             for (int k = 0; k < IT; ++k) {
@@ -121,7 +144,7 @@ __global__ void __launch_bounds__(NTH) k_plane(PlaneArgs a)
             for (int k = 0; k < IT; ++k) {
                 const int e = tid + k * NTH;
                 if (E % NTH == 0 || e < E)
-                    dst[e] = w[k];
+                    dst[at(e)] = w[k];
             }
         }
         __syncthreads();  // the plane is read out before the next one is written

@@ -158,10 +158,10 @@ constexpr size_t plane_lds()
 // fit on the device, each looping over its share of the planes, so that loads of the next
 // plane overlap the transform of the current one); smaller planes run one per workgroup,
 // several workgroups per CU overlapping each other.
-template <bool INV, int TA, int TB, int NTH, class RA, class RB>
-static int launch_plane(const PlaneArgs &a, hipStream_t s)
+template <bool INV, int TA, int TB, int NTH, class RA, class RB, bool SLAB>
+static int launch_plane_as(const PlaneArgs &a, hipStream_t s)
 {
-    const void *fn = (const void *)k_plane<INV, TA, TB, NTH, RA, RB>;
+    const void *fn = (const void *)k_plane<INV, TA, TB, NTH, RA, RB, SLAB>;
     constexpr size_t lds = plane_lds<RA, RB>();
     static long long resident = 0;  // 0: one workgroup per plane
     static std::once_flag once;
@@ -178,8 +178,21 @@ static int launch_plane(const PlaneArgs &a, hipStream_t s)
     const long long grid = (resident > 0 && a.planes > resident) ? resident : a.planes;
     if (grid > 0x7fffffffLL)
         return -2;
-    hipLaunchKernelGGL((k_plane<INV, TA, TB, NTH, RA, RB>), dim3((unsigned)grid), dim3(NTH), lds, s, a);
+    hipLaunchKernelGGL((k_plane<INV, TA, TB, NTH, RA, RB, SLAB>), dim3((unsigned)grid), dim3(NTH), lds, s, a);
     return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+// contiguous planes through the plain instance; slabs through the slab instance when the
+// variant has one (else -2)
+template <bool INV, int TA, int TB, int NTH, class RA, class RB, bool SLAB>
+static int launch_plane(const PlaneArgs &a, hipStream_t s)
+{
+    constexpr int A = RA::N(), B = RB::N();
+    if (a.nsub == 1 && a.rs == B && a.po == (long long)A * B)
+        return launch_plane_as<INV, TA, TB, NTH, RA, RB, false>(a, s);
+    if constexpr (SLAB)
+        return launch_plane_as<INV, TA, TB, NTH, RA, RB, true>(a, s);
+    return -2;
 }
 
 template <class RS>
@@ -188,7 +201,7 @@ static std::vector<int> radix_list()
     return std::vector<int>(RS::r, RS::r + RS::K);
 }
 
-template <int TA, int TB, int NTH, class RA, class RB>
+template <int TA, int TB, int NTH, class RA, class RB, bool SLAB = true>
 static void add_plane(std::vector<PlaneVariant> &v, const char *name)
 {
     constexpr int B = RB::N();
@@ -208,7 +221,8 @@ static void add_plane(std::vector<PlaneVariant> &v, const char *name)
         f.tw_pad = 2;
         f.lds = lds;
         f.name = name;
-        f.launch = inv ? &launch_plane<true, TA, TB, NTH, RA, RB> : &launch_plane<false, TA, TB, NTH, RA, RB>;
+        f.slab = SLAB;
+        f.launch = inv ? &launch_plane<true, TA, TB, NTH, RA, RB, SLAB> : &launch_plane<false, TA, TB, NTH, RA, RB, SLAB>;
         v.push_back(f);
     }
 }
@@ -222,8 +236,14 @@ const std::vector<PlaneVariant> &plane_registry()
         // passes on the 64x96x120 x16 array, which fits in the Infinity Cache (108.7 against
         // 101.1 us), and 3 % faster on 96x120 x1024: not registered.
         add_plane<8, 8, 512, Radices<2, 8, 8>, Radices<2, 8, 8>>(*v, "plane64x64_t8x8");
-        add_plane<8, 8, 1024, Radices<2, 16, 8>, Radices<2, 16, 8>>(*v, "plane128x128_t8x8");
+        // (no slab instance: its addressing spills registers at 1024 threads)
+        add_plane<8, 8, 1024, Radices<2, 16, 8>, Radices<2, 16, 8>, false>(*v, "plane128x128_t8x8");
         add_plane<4, 4, 128, Radices<2, 8, 4>, Radices<2, 8, 4>>(*v, "plane32x32_t4x4");
+        // 64 x 120 (63 KB): the slab of the first and last axes of the 64 x 96 x 120 array,
+        // whose 96 x 120 plane does not fit twice in a CU; two passes instead of three,
+        // 103-111 us against 109-119 us for the three axis passes (x16, alternating runs)
+        add_plane<8, 16, 1024, Radices<2, 8, 8>, Radices<2, 8, 5, 3>>(*v, "plane64x120_t8x16");
+        add_plane<4, 8, 512, Radices<2, 8, 8>, Radices<2, 8, 5, 3>>(*v, "plane64x120_t4x8");
         return v;
     }();
     return *reg;

@@ -166,14 +166,15 @@ def test_nd_in_place_single_call():
 def plane_registry():
     src = open(os.path.join(ROOT, "kissfft_amd", "csrc", "engine", "kfa_tile_reg.hip")).read()
     out = []
-    for m in re.finditer(r"add_plane<\d+, \d+, \d+, Radices<2, ([\d, ]+)>, Radices<2, ([\d, ]+)>>\(\*v, \"(\w+)\"\)", src):
+    for m in re.finditer(r"add_plane<\d+, \d+, \d+, Radices<2, ([\d, ]+)>, Radices<2, ([\d, ]+)>(, false)?>\(\*v, \"(\w+)\"\)", src):
         A = int(np.prod([int(v) for v in m.group(1).split(",")]))
         B = int(np.prod([int(v) for v in m.group(2).split(",")]))
-        out.append((A, B, m.group(3)))
+        out.append((A, B, m.group(4), m.group(3) is None))
     return out
 
 
-PLANES = plane_registry()
+PLANES = [p[:3] for p in plane_registry()]
+SLABS = [p[:3] for p in plane_registry() if p[3]]
 
 
 def test_plane_registry_parsed():
@@ -202,6 +203,31 @@ def test_float_nd_plane(A, B, name, lead, monkeypatch):
         lib.L.kiss_fft_cleanup()
         got = cplx(nd(lib, dims, x))
         assert rel_l2(got, np.fft.fftn(cplx(x), axes=axes)) <= ABS_FLOAT_BOUND, dims
+    finally:
+        lib.L.kiss_fft_cleanup()
+
+
+@pytest.mark.parametrize("A,B,name", SLABS, ids=[p[2] for p in SLABS])
+@pytest.mark.parametrize("shape", ["a_m_b", "l_a_m_b", "a_mm_b"])
+def test_float_nd_slab(A, B, name, shape, monkeypatch, capfd):
+    """A plane variant A x B used as a slab of two non-adjacent axes (the outer axis of
+    length A and the last axis of length B, the axes between them fixed per plane), under
+    a leading axis or none, both directions, against numpy."""
+    dims = {"a_m_b": (A, 3, B), "l_a_m_b": (2, A, 3, B), "a_mm_b": (A, 2, 5, B)}[shape]
+    lib = kissfft_amd.load("float")
+    x = W.complex_pm1(48, int(np.prod(dims)), 2).reshape((2,) + dims + (2,))
+    axes = tuple(range(1, len(dims) + 1))
+    monkeypatch.setenv("KISSFFT_AMD_TILE", name)
+    monkeypatch.setenv("KISSFFT_AMD_VERBOSE", "1")
+    lib.L.kiss_fft_cleanup()
+    try:
+        for inv in (False, True):
+            say("case nd slab", name, dims, inv)
+            capfd.readouterr()
+            want = np.fft.ifftn(cplx(x), axes=axes) * np.prod(dims) if inv else np.fft.fftn(cplx(x), axes=axes)
+            got = cplx(nd(lib, dims, x, inv))
+            assert f"plane {A}x{B} inverse={int(inv)}: {name}" in capfd.readouterr().err, "slab pass not used"
+            assert rel_l2(got, want) <= ABS_FLOAT_BOUND, (dims, inv)
     finally:
         lib.L.kiss_fft_cleanup()
 
