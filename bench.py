@@ -301,14 +301,14 @@ def other_configs(lib, stream, dev) -> dict:
 
     # fast FIR (overlap-scrap, kiss_fastfir): 2^28 real samples (1 GiB) through a 1000-tap
     # filter, one call on device buffers, with the reference's default block (the next power
-    # of two >= 2 (taps - 1): 2048, each sample read 2048 / 1049 times) and with 8192
+    # of two >= 2 (taps - 1): 2048, each sample read 2048 / 1049 times) and with 4096 and 8192
     nsamp, ntap = 1 << 28, 1000
     h = np.random.default_rng(13).standard_normal(ntap).astype(np.float32)
     with torch.cuda.stream(stream):
         fin = torch.rand(nsamp, device=dev) * 2 - 1
         fout = torch.empty(nsamp, device=dev)
     torch.cuda.synchronize()
-    for nfft in (0, 8192):
+    for nfft in (0, 4096, 8192):
         ff = lib.fastfir_alloc(h, real=True, nfft=nfft)
         written = [0]
 
