@@ -44,6 +44,21 @@ inline PlaneArgs plane_args(const void *in, void *out, const void *stw_a, const 
     return PlaneArgs{in, out, stw_a, stw_b, planes, (int)B, 1, A * B, 0};
 }
 
+// Orders the LDS accesses of one line's threads around a stage: when a line's T threads
+// are lanes of one wavefront (T divides 64), only that wavefront touches the line, and the
+// in-order LDS operations of a wavefront need no workgroup barrier.
+template <int T>
+__device__ __forceinline__ void line_sync()
+{
+    if constexpr (64 % T == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
 template <bool INV, int T, class RS, int ES, int S>
 __device__ __forceinline__ void plane_stage(F32::store2 *lb, const F32::store2 *stw, int t, bool active)
 {
@@ -51,12 +66,12 @@ __device__ __forceinline__ void plane_stage(F32::store2 *lb, const F32::store2 *
     typename F::C v[F::J][F::p];
     if (active)
         F::template gather<false>(v, lb, ES, t, true);
-    __syncthreads();  // every input read before the buffer is overwritten
+    line_sync<T>();  // every input read before the buffer is overwritten
     if (active) {
         F::compute(v, stw, t);
         F::template store<false>(v, lb, ES, t, true);
     }
-    __syncthreads();
+    line_sync<T>();
 }
 
 template <bool INV, int T, class RS, int ES, int... S>
@@ -126,11 +141,15 @@ __global__ void __launch_bounds__(NTH) k_plane(PlaneArgs a)
             plane_stages<INV, TB, RB, 1>(std::make_integer_sequence<int, RB::K>{},
                                          buf + (line < A ? line : 0) * RSX, twb, tid - line * TB, line < A);
         }
+        if constexpr (64 % TB == 0)
+            __syncthreads();  // every row done before the columns read them
         {   // columns: A points at stride RSX, TA threads each
             const int line = tid / TA;
             plane_stages<INV, TA, RA, RSX>(std::make_integer_sequence<int, RA::K>{}, buf + (line < B ? line : 0),
                                            twa, tid - line * TA, line < B);
         }
+        if constexpr (64 % TA == 0)
+            __syncthreads();  // every column done before the plane is read out
         {
             St *dst = out + base(pl);
             St w[IT];
