@@ -336,6 +336,21 @@ __device__ __forceinline__ void merge_pair(Cx<A> xk, Cx<A> xnk, Cx<A> w, Cx<A> &
 // give thread t the butterflies t and NB-t (thread 0: 0 and NB/2), so the thread holds
 // both z[k] and z[N-k] of every bin pair its outputs (inputs) touch and the real split
 // (merge) runs in registers, fused with the stage.
+// Orders the LDS accesses of one line's threads around a stage: when a line's T threads
+// are lanes of one wavefront (T divides 64), only that wavefront touches the line, and the
+// in-order LDS operations of a wavefront need no workgroup barrier.
+template <int T>
+__device__ __forceinline__ void line_sync()
+{
+    if constexpr (64 % T == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
 template <class A, bool INV, int T, class RS, int S, bool PAIRED>
 struct FastStage {
     typedef typename A::store2 St;
@@ -599,14 +614,14 @@ __device__ __forceinline__ void fast_run_one(typename A::store2 *bufA, typename 
     else
         F::template gather<GSRC>(v, src, GSRC ? in_el : 1, t, valid);
     if constexpr (!DB && !GSRC && !GDST)
-        __syncthreads();  // every input read before the buffer is overwritten
+        line_sync<T>();  // every input read before the buffer is overwritten
     F::compute(v, stw, t);
     if constexpr (SPLIT)
         F::store_split(v, gout, out_el, rtw, t, valid);
     else
         F::template store<GDST>(v, dst, GDST ? out_el : 1, t, valid);
     if constexpr (!GDST)
-        __syncthreads();
+        line_sync<T>();
 }
 
 // Stages 0 and 1 without the LDS exchange between them.  Thread t owns the stage-0
@@ -650,7 +665,7 @@ __device__ __forceinline__ void fast_run_first_pair(typename A::store2 *bufA, ty
     // stages write bufA), which stage 2 reads; stage 1 is not the last stage (RS::K > 2)
     F1::template store<false>(v, bufA, 1, t, valid);
     (void)bufB;
-    __syncthreads();
+    line_sync<T>();
 }
 
 template <class A, bool INV, int T, int KIND, bool DB, bool FUSE, class RS, int S, int FROM>
@@ -746,7 +761,7 @@ __device__ __forceinline__ void k_fast_body(const FastArgs &a)
                 bufA[k] = O::pack(O::cadd(e, o));
             bufA[N - k] = O::pack(lo);
         }
-        __syncthreads();
+        line_sync<T>();
     }
 
     fast_run_all<A, INV, T, KIND, DB, FUSE, RS>(std::make_integer_sequence<int, K>{}, bufA, bufB, gin, ie,

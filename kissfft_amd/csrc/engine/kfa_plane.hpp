@@ -44,21 +44,6 @@ inline PlaneArgs plane_args(const void *in, void *out, const void *stw_a, const 
     return PlaneArgs{in, out, stw_a, stw_b, planes, (int)B, 1, A * B, 0};
 }
 
-// Orders the LDS accesses of one line's threads around a stage: when a line's T threads
-// are lanes of one wavefront (T divides 64), only that wavefront touches the line, and the
-// in-order LDS operations of a wavefront need no workgroup barrier.
-template <int T>
-__device__ __forceinline__ void line_sync()
-{
-    if constexpr (64 % T == 0) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    } else {
-        __syncthreads();
-    }
-}
-
 template <bool INV, int T, class RS, int ES, int S>
 __device__ __forceinline__ void plane_stage(F32::store2 *lb, const F32::store2 *stw, int t, bool active)
 {

@@ -44,10 +44,10 @@ __device__ __forceinline__ void tile_stage(typename A::store2 *lb, const typenam
     typedef FastStage<A, INV, T, RS, S, false> F;
     typename F::C v[F::J][F::p];
     F::template gather<false>(v, lb, 1, t, true);
-    __syncthreads();  // every input read before the buffer is overwritten
+    line_sync<T>();  // every input read before the buffer is overwritten
     F::compute(v, stw, t);
     F::template store<false>(v, lb, 1, t, true);
-    __syncthreads();
+    line_sync<T>();
 }
 
 template <class A, bool INV, int T, class RS, int... S>
@@ -142,6 +142,8 @@ __global__ void __launch_bounds__(T *TL) k_tile(TileArgs a)
         tile_stages<A, INV, T, RS>(std::make_integer_sequence<int, RS::K>{}, buf + line * NS,
                                    reinterpret_cast<const St *>(a.stw), t);
     }
+    if constexpr (64 % T == 0)
+        __syncthreads();  // every line done before the store phase reads across lines
 
     {
         const int ln = line_of(SCOL), e0 = elem0(SCOL);

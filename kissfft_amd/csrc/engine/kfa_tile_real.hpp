@@ -162,6 +162,8 @@ __global__ void __launch_bounds__(T *TL) k_tile_rsplit(RealTileArgs a)
         tile_stages<F32, false, T, RS>(std::make_integer_sequence<int, RS::K>{}, buf + line * NS,
                                        reinterpret_cast<const St *>(a.stw), t);
     }
+    if constexpr (64 % T == 0)
+        __syncthreads();  // the stages order each line only: the split reads across lines
     St *out = reinterpret_cast<St *>(a.out) + b * a.out_hi;
 #pragma unroll
     for (int i = 0; i < IS; ++i) {
@@ -329,6 +331,8 @@ __global__ void __launch_bounds__(T *TL) k_tile_rmerge(RealTileArgs a)
         tile_stages<F32, true, T, RS>(std::make_integer_sequence<int, RS::K>{}, buf + line * NS,
                                       reinterpret_cast<const St *>(a.stw), t);
     }
+    if constexpr (64 % T == 0)
+        __syncthreads();  // the stages order each line only: the stores read across lines
     {   // column stores with W^(col*k2), k2 = e0 + k*T
         const int s = tid % TL, e0 = tid / TL;
         const C base = O::cmul(O::load(bh), O::load(bl));
