@@ -807,13 +807,14 @@ static int launch_tile(const TileChoice &tc, TileArgs a, hipStream_t s)
     a.stw = tc.d_stw;
     const long long tiles = (a.nlines + tc.v->TL - 1) / tc.v->TL;
     // Column phases (consecutive lines = neighbouring column segments of the same rows):
-    // groups of 8 neighbouring tiles run on one XCD.  Segments narrower than a 128-byte
+    // groups of 16 neighbouring tiles run on one XCD.  Segments narrower than a 128-byte
     // line, or rows that do not start on a line (spectra of nfft/2+1 bins), share lines
     // with the neighbouring tile, which then hit in that XCD's L2; aligned segments gain
-    // from neighbours sharing DRAM pages (measured: 2-D real 1024x1000 +13 % forward and
-    // +17 % inverse, c2c 2^20 +10 %).
+    // from neighbours sharing DRAM pages (measured against no grouping: 2-D real 1024x1000
+    // +13 % forward and +17 % inverse, c2c 2^20 +10 % with groups of 8; groups of 16
+    // against 8: c2c / r2c / c2r 2^20 +3 / +5 / +2 %, 2-D real +1-2 %; 32: no further gain).
     const bool cols = a.in_lo == 1 || a.out_lo == 1;
-    a.xcd_group = cols ? 8 : 1;
+    a.xcd_group = cols ? 16 : 1;
     if (tiles > 0x7fffffffLL)
         return fail(-2, "too many lines (%lld) for one launch", a.nlines);
     ++g_launches;
@@ -832,9 +833,9 @@ static int launch_rtile(const TileChoice &tc, RealTileArgs a, long long howmany,
     if (blocks > 0x7fffffffLL)
         return fail(-2, "too many transforms (%lld) for one launch", howmany);
     a.tiles = blocks;
-    // neighbouring tiles share the cache lines of their bin runs and DRAM pages (as for
-    // the column phases of launch_tile)
-    a.xcd_group = 8;
+    // neighbouring tiles share the cache lines of their bin runs and DRAM pages (groups of
+    // 16, as for the column phases of launch_tile)
+    a.xcd_group = 16;
     ++g_launches;
     return tc.v->launch_real(a, blocks, s) ? fail(-3, "fused real tile kernel launch failed") : 0;
 }
