@@ -1461,6 +1461,15 @@ static int launch_dft_stage(int type, const DftStageArgs &a, hipStream_t s)
     }
 }
 
+// Chunks of a batch that does not fit one temporary: as many as the room requires, of
+// nearly equal size (256 transforms at 127 per chunk run as 86 + 86 + 84, not 127 + 127 + 2:
+// a launch of a few transforms costs almost as much as a full one).
+static long long balanced_chunk(long long howmany, long long max_chunk)
+{
+    const long long n = (howmany + max_chunk - 1) / max_chunk;
+    return (howmany + n - 1) / n;
+}
+
 size_t multipass_bytes_per_transform(DevPlan &P, int fk)
 {
     if (P.single)
@@ -1490,8 +1499,9 @@ int run_c2c(DevPlan &P, const void *in, long long istride, long long idist, void
         const size_t room = scratch_room();
         if (room < per)
             return fail(-2, "nfft=%d needs %zu bytes of scratch per transform; %zu available", n, per, room);
-        const long long chunk = std::max(1LL, (long long)(room / per));
+        long long chunk = std::max(1LL, (long long)(room / per));
         if (howmany > chunk) {
+            chunk = balanced_chunk(howmany, chunk);
             for (long long b = 0; b < howmany; b += chunk) {
                 const int rc = run_c2c(P, (const char *)in + (size_t)(b * idist) * cb, istride, idist,
                                        (char *)out + (size_t)(b * odist) * cb, ostride, odist,
@@ -1814,8 +1824,9 @@ static int run_real_multipass(DevPlan &H, const RealTables &R, int inverse, cons
     const size_t room = scratch_room();
     if (room < per)
         return fail(-2, "real nfft=%d needs %zu bytes of scratch per transform; %zu available", 2 * nc, per, room);
-    const long long chunk = std::max(1LL, (long long)(room / per));
+    long long chunk = std::max(1LL, (long long)(room / per));
     if (howmany > chunk) {  // bounded temporaries, as for multi-pass complex lines
+        chunk = balanced_chunk(howmany, chunk);
         for (long long b = 0; b < howmany; b += chunk) {
             const int rc = run_real_multipass(H, R, inverse, (const char *)in + (size_t)(b * in_hi) * cb, in_hi,
                                               (char *)out + (size_t)(b * out_hi) * cb, out_hi,
