@@ -77,19 +77,19 @@ static void add_variant(std::vector<FastVariant> &v, int type, const char *name)
     }
 }
 
-template <bool INV, int T, int LPB, class RS, bool ASM>
+template <bool INV, int T, int LPB, class RS>
 static int launch_wave(const FastArgs &a, hipStream_t s)
 {
     const long long blocks = (a.nlines + LPB - 1) / LPB;
     if (blocks <= 0)
         return 0;
-    hipLaunchKernelGGL((k_q15w<INV, T, LPB, RS, ASM>), dim3((unsigned)blocks), dim3(T * LPB), 0, s, a);
+    hipLaunchKernelGGL((k_q15w<INV, T, LPB, RS>), dim3((unsigned)blocks), dim3(T * LPB), 0, s, a);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
 // Q15 complex lines held by one wavefront (kfa_q15w.hpp): stage exchanges by lane swaps,
 // no LDS; radix 2 / 4 power-of-two schedules
-template <int T, int LPB, bool ASM, int... R>
+template <int T, int LPB, int... R>
 static void add_wave(std::vector<FastVariant> &v, const char *name)
 {
     typedef Radices<TwPad<Q15P>::G, R...> RS;
@@ -107,7 +107,7 @@ static void add_wave(std::vector<FastVariant> &v, const char *name)
         f.tw_fmt = TwPad<Q15P>::fmt;
         f.tw_len = RS::tw_len();
         f.name = name;
-        f.launch = inv ? &launch_wave<true, T, LPB, RS, ASM> : &launch_wave<false, T, LPB, RS, ASM>;
+        f.launch = inv ? &launch_wave<true, T, LPB, RS> : &launch_wave<false, T, LPB, RS>;
         v.push_back(f);
     }
 }
@@ -238,39 +238,27 @@ const std::vector<FastVariant> &fast_registry()
         add_variant<F32, FK_C2C, 125, 2, false, false, 5, 5, 5, 4>(*v, KFA_F32, "c2c500_5554_t125x2_sb");
         add_variant<F32, FK_C2C, 192, 1, false, false, 3, 16, 8, 8>(*v, KFA_F32, "c2c3072_31688_t192_sb");
         add_variant<F32, FK_C2C, 256, 1, false, false, 16, 16, 16>(*v, KFA_F32, "c2c4096_161616_t256_sb");
-        // ---- Q15 (reference schedules), wavefront-resident lines: lane-swap exchanges ----
+        // ---- Q15 (reference schedules), wavefront-resident lines: one LDS transpose ----
         add_xpose<8, 1, 2, 4, 4, 4, 4>(*v, "q15x_c2c512_t16x8");
         add_xpose<16, 1, 2, 4, 4, 4, 4>(*v, "q15x_c2c512_t16x16");
         add_xpose<8, 6, 2, 4, 4, 4, 4>(*v, "q15x_c2c512_t16x8_o6");
         add_xpose<4, 6, 2, 4, 4, 4, 4>(*v, "q15x_c2c512_t16x4_o6");
         add_xpose<16, 1, 4, 4, 4, 4>(*v, "q15x_c2c256_t16x16");
-        add_wave<16, 16, true, 2, 4, 4, 4, 4>(*v, "q15w_c2c512_t16x16_asm");
-        add_wave<32, 8, true, 2, 4, 4, 4, 4>(*v, "q15w_c2c512_t32x8_asm");
-        add_wave<16, 16, false, 2, 4, 4, 4, 4>(*v, "q15w_c2c512_t16x16");
-        add_wave<32, 8, false, 2, 4, 4, 4, 4>(*v, "q15w_c2c512_t32x8");
-        add_wave<16, 8, false, 2, 4, 4, 4, 4>(*v, "q15w_c2c512_t16x8");
-        add_wave<16, 4, false, 2, 4, 4, 4, 4>(*v, "q15w_c2c512_t16x4");
-        add_wave<64, 4, false, 2, 4, 4, 4, 4>(*v, "q15w_c2c512_t64x4");
-        add_wave<32, 4, false, 2, 4, 4, 4, 4>(*v, "q15w_c2c512_t32x4");
-        add_wave<16, 16, false, 4, 4, 4, 4>(*v, "q15w_c2c256_t16x16");
-        add_wave<32, 8, false, 4, 4, 4, 4, 4>(*v, "q15w_c2c1024_t32x8");
-        add_wave<16, 16, false, 4, 4, 4, 4, 4>(*v, "q15w_c2c1024_t16x16");
-        add_wave<64, 4, false, 2, 4, 4, 4, 4, 4>(*v, "q15w_c2c2048_t64x4");
-        // ---- Q15 (reference schedules), packed-word butterflies ----
-        add_variant<Q15P, FK_C2C, 32, 8, false, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c512_t32x8_sb");
+        // ---- Q15 (reference schedules), wavefront-resident lines: lane-swap exchanges ----
+        add_wave<16, 16, 2, 4, 4, 4, 4>(*v, "q15w_c2c512_t16x16");
+        add_wave<32, 8, 2, 4, 4, 4, 4>(*v, "q15w_c2c512_t32x8");
+        add_wave<16, 16, 4, 4, 4, 4>(*v, "q15w_c2c256_t16x16");
+        add_wave<32, 8, 4, 4, 4, 4, 4>(*v, "q15w_c2c1024_t32x8");
+        add_wave<16, 16, 4, 4, 4, 4, 4>(*v, "q15w_c2c1024_t16x16");
+        add_wave<64, 4, 2, 4, 4, 4, 4, 4>(*v, "q15w_c2c2048_t64x4");
+        // ---- Q15 (reference schedules), packed-word butterflies through LDS ----
         add_variant<Q15P, FK_C2C, 16, 16, false, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c512_t16x16_sb");
-        add_variant<Q15P, FK_C2C, 32, 8, true, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c512_t32x8_db");
-        add_variant<Q15P, FK_C2C, 64, 4, false, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c512_t64x4_sb");
-        add_variant<Q15P, FK_C2C, 64, 4, true, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c512_t64x4_db");
-        add_variant<Q15P, FK_C2C, 128, 2, false, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c512_t128x2_sb");
+        add_variant<Q15P, FK_C2C, 32, 8, false, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c512_t32x8_sb");
         add_variant<Q15P, FK_C2C, 240, 1, false, false, 5, 5, 3, 4, 4>(*v, KFA_Q15, "q15p_c2c1200_t240_sb");
         add_variant<Q15P, FK_C2C, 64, 4, false, false, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c256_t64x4_sb");
         add_variant<Q15P, FK_C2C, 128, 2, false, false, 4, 4, 4, 4, 4>(*v, KFA_Q15, "q15p_c2c1024_t128x2_sb");
-        // ---- Q15 (reference schedules) ----
-        add_variant<Q15, FK_C2C, 128, 2, true, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15c2c512_t128x2_db");
+        // ---- Q15 (reference schedules), one value per component (the real kinds' arithmetic) ----
         add_variant<Q15, FK_C2C, 128, 2, false, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15c2c512_t128x2_sb");
-        add_variant<Q15, FK_C2C, 64, 4, false, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15c2c512_t64x4_sb");
-        add_variant<Q15, FK_C2C, 128, 4, false, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15c2c512_t128x4_sb");
         add_real<Q15, 128, 2, false, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15real512_t128x2_sb");
         add_real<Q15, 256, 1, false, 2, 4, 4, 4, 4, 4>(*v, KFA_Q15, "q15real2048_t256_sb");
         add_fused<Q15, FK_R2C, 64, 4, 2, 4, 4, 4, 4>(*v, KFA_Q15, "q15r2c512_t64x4_fused");

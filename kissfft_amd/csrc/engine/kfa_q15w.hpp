@@ -145,71 +145,10 @@ namespace q15w {
 
 typedef unsigned u32;
 
-// Lanes whose bit L is clear (L < 4), as a wave-wide mask.
-template <int L>
-constexpr unsigned long long lo_lanes()
-{
-    unsigned long long m = 0;
-    for (int l = 0; l < 64; ++l)
-        if (!((l >> L) & 1))
-            m |= 1ULL << l;
-    return m;
-}
-
-// Lane bit L (< 4) swapped with a register bit for two register pairs (A0, B0), (A1, B1):
-// each new value is one v_cndmask_b32 with a DPP source (the partner lane's value) and the
-// lane mask in VCC -- na = lo ? A : B[lane - 2^L], nb = hi ? B : A[lane + 2^L].  The two
-// wait states a DPP read needs after a vector write of its source come first; bound_ctrl:0
-// makes lanes whose partner is outside the row read 0 (they select their own value).
-template <int L>
-__device__ __forceinline__ void dpp_swap2(u32 &A0, u32 &B0, u32 &A1, u32 &B1)
-{
-    static_assert(L >= 0 && L < 4, "DPP row shifts reach lane bits 0-3");
-    u32 na0, na1, nb0, nb1;
-#define KFA_Q15W_SWAP_ASM(SH)                                                                          \
-    asm("s_mov_b64 vcc, %8\n\t"                                                                           \
-        "s_nop 1\n\t"                                                                                     \
-        "v_cndmask_b32_dpp %0, %5, %4, vcc row_shr:" #SH " row_mask:0xf bank_mask:0xf bound_ctrl:0\n\t"  \
-        "v_cndmask_b32_dpp %1, %7, %6, vcc row_shr:" #SH " row_mask:0xf bank_mask:0xf bound_ctrl:0\n\t"  \
-        "s_not_b64 vcc, vcc\n\t"                                                                          \
-        "v_cndmask_b32_dpp %2, %4, %5, vcc row_shl:" #SH " row_mask:0xf bank_mask:0xf bound_ctrl:0\n\t"  \
-        "v_cndmask_b32_dpp %3, %6, %7, vcc row_shl:" #SH " row_mask:0xf bank_mask:0xf bound_ctrl:0"        \
-        : "=&v"(na0), "=&v"(na1), "=&v"(nb0), "=&v"(nb1)                                                  \
-        : "v"(A0), "v"(B0), "v"(A1), "v"(B1), "s"(lo_lanes<L>())                                          \
-        : "vcc")
-    if constexpr (L == 0)
-        KFA_Q15W_SWAP_ASM(1);
-    else if constexpr (L == 1)
-        KFA_Q15W_SWAP_ASM(2);
-    else if constexpr (L == 2)
-        KFA_Q15W_SWAP_ASM(4);
-    else
-        KFA_Q15W_SWAP_ASM(8);
-#undef KFA_Q15W_SWAP_ASM
-    A0 = na0;
-    A1 = na1;
-    B0 = nb0;
-    B1 = nb1;
-}
-
 // Swaps lane bit L with register bit J across the R registers of a lane.
-template <int L, int J, int R, bool ASM>
+template <int L, int J, int R>
 __device__ __forceinline__ void swap_bit(u32 (&v)[R], int lane)
 {
-    if constexpr (ASM && L < 4) {
-        // pairs (x, x | 2^J) taken two at a time
-        constexpr int NP = R / 2;
-        static_assert(NP % 2 == 0, "an even number of register pairs");
-        int xs[NP] = {};
-        int k = 0;
-        for (int x = 0; x < R; ++x)
-            if (!(x & (1 << J)))
-                xs[k++] = x;
-#pragma unroll
-        for (int i = 0; i < NP; i += 2)
-            dpp_swap2<L>(v[xs[i]], v[xs[i] | (1 << J)], v[xs[i + 1]], v[xs[i + 1] | (1 << J)]);
-        return;
-    }
 #pragma unroll
     for (int x = 0; x < R; ++x) {
         if (x & (1 << J))
@@ -227,9 +166,9 @@ __device__ __forceinline__ void swap_bit(u32 (&v)[R], int lane)
             B = r[1];
         } else {
             // lanes with bit L set take B of lane - 2^L (row_shr), the others A of lane + 2^L
-            // (row_shl); both partners sit in the lane's own row of 16.  Written as selects
-            // whose moved operand is the false one, so each folds into one v_cndmask with a
-            // DPP source: one instruction per new value, no copies.
+            // (row_shl); both partners sit in the lane's own row of 16: a DPP move and a
+            // select per new value (the compiler keeps them apart: the select's lane mask
+            // is a scalar register pair, which the DPP form of v_cndmask cannot take).
             const bool hi = (lane >> L) & 1;
             const u32 fromB = (u32)__builtin_amdgcn_mov_dpp((int)B, 0x110 + (1 << L), 0xf, 0xf, true);
             const u32 fromA = (u32)__builtin_amdgcn_mov_dpp((int)A, 0x100 + (1 << L), 0xf, 0xf, true);
@@ -241,12 +180,12 @@ __device__ __forceinline__ void swap_bit(u32 (&v)[R], int lane)
     }
 }
 
-template <class PL, int S, int K, int R, bool ASM>
+template <class PL, int S, int K, int R>
 __device__ __forceinline__ void swaps(u32 (&v)[R], int lane)
 {
     if constexpr (K < PL::d.nswap[S]) {
-        swap_bit<PL::d.swap_lane[S][K], PL::d.swap_reg[S][K], R, ASM>(v, lane);
-        swaps<PL, S, K + 1, R, ASM>(v, lane);
+        swap_bit<PL::d.swap_lane[S][K], PL::d.swap_reg[S][K], R>(v, lane);
+        swaps<PL, S, K + 1, R>(v, lane);
     }
 }
 
@@ -279,11 +218,11 @@ __device__ __forceinline__ int lane_field(int lane, int lo, int w)
 
 // Stage S: swaps, then every butterfly of the lane (its inputs are the registers whose
 // butterfly bits vary; outputs replace them in place), twiddle rows u from the stage table.
-template <bool INV, class RS, class PL, int S, bool ASM>
+template <bool INV, class RS, class PL, int S>
 __device__ __forceinline__ void stage(u32 (&v)[PL::R], const uint2 *__restrict__ tab, int lane)
 {
     constexpr int R = PL::R;
-    swaps<PL, S, 0, R, ASM>(v, lane);
+    swaps<PL, S, 0, R>(v, lane);
     constexpr int p = RS::p(S);
     constexpr int c = ilog2(p);
     constexpr int m = RS::m(S);
@@ -351,10 +290,10 @@ __device__ __forceinline__ void stage(u32 (&v)[PL::R], const uint2 *__restrict__
     }
 }
 
-template <bool INV, class RS, class PL, bool ASM, int... S>
+template <bool INV, class RS, class PL, int... S>
 __device__ __forceinline__ void stages(std::integer_sequence<int, S...>, u32 (&v)[PL::R], const uint2 *tab, int lane)
 {
-    (stage<INV, RS, PL, S, ASM>(v, tab, lane), ...);
+    (stage<INV, RS, PL, S>(v, tab, lane), ...);
 }
 
 }  // namespace q15w
@@ -558,7 +497,7 @@ __device__ __forceinline__ void xstages(std::integer_sequence<int, S...>, u32 (&
             for (int x = 0; x < PL::R; ++x)
                 v[x] = w[x];
         }
-        stage<INV, RS, PL, s, false>(v, tab, lane);
+        stage<INV, RS, PL, s>(v, tab, lane);
     };
     (one(std::integral_constant<int, S>{}), ...);
 }
@@ -614,9 +553,7 @@ __global__ void __launch_bounds__(16 * LPB, WPE) k_q15x(FastArgs a)
 }
 
 // Q15 c2c lines of the power-of-two schedule RS, T lanes per line, LPB lines per workgroup.
-// ASM: the lane-bit 0-3 swaps as hand-placed v_cndmask_b32 with DPP sources (else the
-// compiler's moves and selects).
-template <bool INV, int T, int LPB, class RS, bool ASM>
+template <bool INV, int T, int LPB, class RS>
 __global__ void __launch_bounds__(T *LPB) k_q15w(FastArgs a)
 {
     typedef WavePlan<RS, T> PL;
@@ -646,8 +583,8 @@ __global__ void __launch_bounds__(T *LPB) k_q15w(FastArgs a)
         for (int x = 0; x < R; ++x)
             v[x] = __builtin_bit_cast(u32, hbm_ld(gin + (long long)(lane + T * x) * ie));
     }
-    q15w::stages<INV, RS, PL, ASM>(std::make_integer_sequence<int, RS::K>{}, v, tab, lane);
-    q15w::swaps<PL, RS::K, 0, R, ASM>(v, lane);
+    q15w::stages<INV, RS, PL>(std::make_integer_sequence<int, RS::K>{}, v, tab, lane);
+    q15w::swaps<PL, RS::K, 0, R>(v, lane);
     if (!valid)
         return;
     const int y_lane = q15w::lane_field<PL, RS::K>(lane, 0, PL::NBITS);

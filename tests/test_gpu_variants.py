@@ -30,8 +30,8 @@ def registry():
     for m in re.finditer(r"add_(real|variant|fused|wave|xpose)<([^>]*)>\(\*v,\s*(?:KFA_\w+,\s*)?\"(\w+)\"\)", src):
         fn, targs, name = m.groups()
         a = [x.strip() for x in targs.split(",")]
-        if fn in ("wave", "xpose"):  # <T, LPB, ASM, radices...> / <LPB, WPE, radices...>: Q15 complex
-            n = int(np.prod([int(r) for r in a[3 if fn == "wave" else 2:]]))
+        if fn in ("wave", "xpose"):  # <T, LPB, radices...> / <LPB, WPE, radices...>: Q15 complex
+            n = int(np.prod([int(r) for r in a[2:]]))
             out.append(("Q15P", "c2c", n, name))
             continue
         if fn == "real":  # <A, T, LPB, DB, radices...>
