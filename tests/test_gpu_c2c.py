@@ -227,18 +227,19 @@ def test_kfc_cached_calls(lf):
 
 
 def test_float_long_lines_chunked_batches(lf):
-    # batches whose four-step temporary exceeds the 1 GiB scratch bound run in chunks
-    # (2^20 complex: 128 transforms per chunk; 2^20 real: 256): sampled transforms against
-    # FP64 across chunk edges, every transform through the round trip
+    # batches whose four-step temporary exceeds the 2 GiB scratch bound run in chunks of
+    # equal size (2^20 points, 260 transforms: two chunks of 130, complex and real): sampled
+    # transforms against FP64 on both sides of the chunk edge, every transform through the
+    # round trip
     import torch
     dev = torch.device("cuda:0")
-    n, B = 1 << 20, 130
+    n, B = 1 << 20, 260
     g = torch.Generator(device=dev).manual_seed(77)
     x = torch.rand(B, n, 2, device=dev, generator=g) * 2 - 1
     y = torch.empty_like(x)
     lf.fft_batch(lf.fft_alloc(n), x, y, B, n, n)
     torch.cuda.synchronize()
-    for b in (0, 127, 128, 129):
+    for b in (0, 129, 130, 259):
         say("chunked c2c sample", b)
         assert rel_l2(cplx(y[b].cpu().numpy()), np.fft.fft(cplx(x[b].cpu().numpy()))) <= ABS_FLOAT_BOUND, b
     z = torch.empty_like(x)
@@ -254,7 +255,7 @@ def test_float_long_lines_chunked_batches(lf):
     X = torch.empty(B, n // 2 + 1, 2, device=dev)
     lf.fftr_batch(lf.fftr_alloc(n), xr, X, B, n, n // 2 + 1)
     torch.cuda.synchronize()
-    for b in (0, 255, 256, 259):
+    for b in (0, 129, 130, 259):
         say("chunked r2c sample", b)
         want = np.fft.rfft(xr[b].cpu().numpy().astype(np.float64))
         assert rel_l2(cplx(X[b].cpu().numpy()), want) <= ABS_FLOAT_BOUND, b
