@@ -203,6 +203,17 @@ def other_configs(lib, stream, dev) -> dict:
     rec("c2c_float_1200_x256", t, 256 * 16 * 1200, 256, us_via_python_binding=round(t_py * 1e6, 1),
         calls="kiss_fft_batch_async back to back on one stream")
 
+    # the same length batched at HBM scale: complex float N=1200, 2^18 transforms (1.2 GiB
+    # per side), and a power-of-two length, N=4096 x 65536 (2 GiB per side)
+    for n, b in ((1200, 1 << 18), (4096, 1 << 16)):
+        with torch.cuda.stream(stream):
+            xb = torch.rand(b, n, 2, device=dev) * 2 - 1
+            yb = torch.empty_like(xb)
+        cb = lib.fft_alloc(n)
+        t = _time(lambda: lib.fft_batch(cb, xb, yb, b, n, n, stream=stream), stream, 10)
+        rec(f"c2c_float_{n}_x{b}", t, b * 16 * n, b, first_call_ms=round(_time.first_s * 1e3, 1))
+        del xb, yb
+
     # Q15 complex N=512, 2^20 transforms
     lq = kissfft_amd.load("int16_t")
     B = 1 << 20
