@@ -80,6 +80,13 @@ def test_forced_variant(typ, kind, n, name, monkeypatch, capfd):
                     y = np.zeros_like(x)
                     lib.fft_batch(cfg, x, y, B, n, n)
                     np.testing.assert_array_equal(y, O.fft(x, inv))
+                # the strided instance: every other input element (in_stride 2), output rows
+                # n + 3 apart; the gaps between output rows keep the caller's bytes
+                xs = rng.integers(lo, hi, (B, 2 * n, 2)).astype(dt)
+                ys = np.full((B, n + 3, 2), 7, dt)
+                lib.fft_batch(lib.fft_alloc(n), xs, ys, B, 2 * n, n + 3, in_stride=2)
+                np.testing.assert_array_equal(ys[:, :n], O.fft(np.ascontiguousarray(xs[:, ::2])))
+                assert (ys[:, n:] == 7).all()
             elif kind == "r2c":
                 x = rng.integers(lo, hi, (B, 2 * n)).astype(dt)
                 y = np.zeros((B, n + 1, 2), dt)
