@@ -16,8 +16,12 @@
  *     without a compiled kernel) joins a single persistent launch that walks a table of
  *     all of them, any mix of lengths and kinds together;
  *   - items longer than one LDS line (float 8192 complex / 16384 real, Q15 16384 / 32768)
- *     run as their own multi-pass launches; a multi-batch holding such items cannot be
- *     captured (they allocate scratch).
+ *     run as their own multi-pass launches, their temporaries in a workspace the
+ *     multi-batch allocates at creation;
+ *   - the compiled-kernel and Bluestein launches of the most common workgroup size are
+ *     fused into one launch of a kernel compiled at run time from their own bodies (one
+ *     grid, the groups in workgroup ranges, largest first); KISSFFT_AMD_MULTI_FUSED=0 at
+ *     creation keeps separate launches.
  * The launches run back to back on the caller's stream and capture records them into one
  * graph (KISSFFT_AMD_MULTI_HELPERS=n at creation forks them over n streams of the
  * multi-batch, joined on the caller's stream; the hand-overs between queues cost more than
@@ -72,6 +76,10 @@ int KISS_FFT_API kiss_fft_multi_info(kiss_fft_multi m, long long *tiles, int *gr
 /* Compiled-kernel launches and the items they cover, Bluestein items, multi-pass items. */
 int KISS_FFT_API kiss_fft_multi_layout(kiss_fft_multi m, int *groups, int *group_items, int *bluestein_items,
                                        int *multipass_items);
+
+/* The fused launch: the compiled-kernel / Bluestein groups it runs, its workgroups and
+ * threads per workgroup (all 0: no fused launch). */
+int KISS_FFT_API kiss_fft_multi_fused(kiss_fft_multi m, int *members, long long *workgroups, int *threads);
 
 void KISS_FFT_API kiss_fft_multi_free(kiss_fft_multi m);
 

@@ -154,6 +154,7 @@ class Library:
             "kiss_fft_multi_info": ([vp, ctypes.POINTER(ll), ctypes.POINTER(ci), ctypes.POINTER(ci)], ci),
             "kiss_fft_multi_layout": ([vp, ctypes.POINTER(ci), ctypes.POINTER(ci), ctypes.POINTER(ci),
                                        ctypes.POINTER(ci)], ci),
+            "kiss_fft_multi_fused": ([vp, ctypes.POINTER(ci), ctypes.POINTER(ll), ctypes.POINTER(ci)], ci),
             "kiss_fft_multi_free": ([vp], None),
             "kiss_fastfir_alloc_cpx": ([vp, ctypes.c_size_t, szp, vp, szp], vp),
             "kiss_fastfir_alloc_real": ([vp, ctypes.c_size_t, szp, vp, szp], vp),
@@ -375,8 +376,11 @@ class MultiBatch:
         gr, gi, bl, mp = (ctypes.c_int() for _ in range(4))
         self.lib.L.kiss_fft_multi_layout(self.ptr, ctypes.byref(gr), ctypes.byref(gi), ctypes.byref(bl),
                                          ctypes.byref(mp))
+        fm, fw, ft = ctypes.c_int(), ctypes.c_longlong(), ctypes.c_int()
+        self.lib.L.kiss_fft_multi_fused(self.ptr, ctypes.byref(fm), ctypes.byref(fw), ctypes.byref(ft))
         return {"tiles": t.value, "grid": g.value, "persistent_items": p.value, "groups": gr.value,
-                "group_items": gi.value, "bluestein_items": bl.value, "multipass_items": mp.value}
+                "group_items": gi.value, "bluestein_items": bl.value, "multipass_items": mp.value,
+                "fused_groups": fm.value, "fused_workgroups": fw.value, "fused_threads": ft.value}
 
     def close(self) -> None:
         if getattr(self, "ptr", None):

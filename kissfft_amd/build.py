@@ -42,8 +42,9 @@ DATATYPES = {
 HOST_SOURCES = ["kf_core.c", "kf_real.c", "kf_nd.c", "kf_ndr.c", "kf_cache.c", "kf_serve.c", "kf_multi.c", "kf_fastfir.c"]
 ENGINE_SOURCES = ["kfa_engine.hip", "kfa_fast_reg.hip", "kfa_tile_reg.hip", "kfa_serve.hip", "kfa_multi.hip", "kfa_conv.hip",
                   "kfa_rtc.hip"]
-# headers the runtime compiler needs for k_fast (kfa_rtc.hip), embedded in the library
-RTC_HEADERS = ["kfa_fast.hpp", "kfa_arith.hpp", "kfa_lines.hpp", "kfa_q15p.hpp"]
+# headers the runtime compiler needs for k_fast and the fused multi-batch launch
+# (kfa_rtc.hip), embedded in the library
+RTC_HEADERS = ["kfa_fast.hpp", "kfa_arith.hpp", "kfa_lines.hpp", "kfa_q15p.hpp", "kfa_tile.hpp", "kfa_blue.hpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CC = os.environ.get("CC", "gcc")

@@ -30,8 +30,9 @@ struct BlueArgs {
     FastSeg seg[KFA_MAX_SEG];
 };
 
+// The Bluestein body for workgroup `blk` of its launch (see k_fast_body).
 template <int T, int LPB, class RS>
-__global__ void __launch_bounds__(T *LPB) k_blue(BlueArgs a)
+__device__ __forceinline__ void k_blue_body(const BlueArgs &a, long long blk)
 {
     typedef F32::store2 St;
     typedef Ops<F32> O;
@@ -42,7 +43,7 @@ __global__ void __launch_bounds__(T *LPB) k_blue(BlueArgs a)
     extern __shared__ __align__(16) unsigned char kfa_blue_smem[];
     const int line = threadIdx.x / T, t = threadIdx.x - line * T;
     St *buf = reinterpret_cast<St *>(kfa_blue_smem) + line * (M + 1);
-    const long long gl = (long long)blockIdx.x * LPB + line;
+    const long long gl = blk * LPB + line;
     const bool ok = gl < a.nlines;
     const int n = a.n;
     const St *chirp = reinterpret_cast<const St *>(a.chirp);
@@ -81,6 +82,12 @@ __global__ void __launch_bounds__(T *LPB) k_blue(BlueArgs a)
         if (i < n)
             dst[(long long)i * a.out_el] = O::pack(O::cmul(O::load(buf[i]), O::load(chirp[i])));
     }
+}
+
+template <int T, int LPB, class RS>
+__global__ void __launch_bounds__(T *LPB) k_blue(BlueArgs a)
+{
+    k_blue_body<T, LPB, RS>(a, blockIdx.x);
 }
 
 }  // namespace kfa

@@ -959,6 +959,12 @@ static void blue_args(const BlueChoice &bc, int n, BlueArgs &b)
     b.n = n;
 }
 
+const BlueVariant *blue_fill(const BlueChoice &bc, int n, BlueArgs &b)
+{
+    blue_args(bc, n, b);
+    return bc.v;
+}
+
 // Whether float plan P runs Bluestein's kernel; its tables are built now rather than at
 // the first launch (the multi-batch decides at creation, before any capture).
 int blue_prepare(DevPlan &P, bool *applies, std::shared_ptr<const BlueChoice> *keep)

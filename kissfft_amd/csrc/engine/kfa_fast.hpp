@@ -89,6 +89,15 @@ struct FastArgs {
     FastSeg seg[KFA_MAX_SEG];    // ascending `first`, seg[0].first == 0 (passed by value)
 };
 
+// The kernel argument of a fused launch (several row-kernel launches of one workgroup size
+// as one grid): member i runs workgroups [end[i-1], end[i]) with the argument block at
+// arg[i] (a FastArgs or a BlueArgs in device memory).
+#define KFA_MAX_FUSED 16
+struct FusedTab {
+    const void *arg[KFA_MAX_FUSED];
+    long long end[KFA_MAX_FUSED];
+};
+
 // HBM accesses of the fast kernels.  KFA_NT selects non-temporal (streaming) loads and
 // stores: every input is read once and every output written once.
 #ifndef KFA_NT
@@ -702,9 +711,11 @@ struct LastBuf {
     static constexpr bool is_a = !DB || (K % 2 == 0);
 };
 
-// The kernel body; UNIT: both element strides are 1.
+// The kernel body for workgroup `blk` of the launch (blockIdx.x of k_fast; a fused launch
+// of several groups passes the workgroup's index within its group); UNIT: both element
+// strides are 1.
 template <class A, int KIND, bool INV, int T, int LPB, bool DB, bool FUSE, class RS, bool UNIT>
-__device__ __forceinline__ void k_fast_body(const FastArgs &a)
+__device__ __forceinline__ void k_fast_body(const FastArgs &a, long long blk)
 {
     typedef typename A::store2 S;
     typedef Ops<A> O;
@@ -714,7 +725,7 @@ __device__ __forceinline__ void k_fast_body(const FastArgs &a)
     extern __shared__ __align__(16) unsigned char kfa_fast_smem[];
     const int line = threadIdx.x / T;
     const int t = threadIdx.x - line * T;
-    const long long gl = (long long)blockIdx.x * LPB + line;
+    const long long gl = blk * LPB + line;
     const bool valid = gl < a.nlines;
     S *bufA = reinterpret_cast<S *>(kfa_fast_smem) + line * N;
     S *bufB = DB ? reinterpret_cast<S *>(kfa_fast_smem) + (LPB + line) * N : bufA;
@@ -819,11 +830,11 @@ __global__ void __launch_bounds__(T *LPB) k_fast(FastArgs a)
 {
     if constexpr (A::fixed) {
         if (a.in_el == 1 && a.out_el == 1) {
-            k_fast_body<A, KIND, INV, T, LPB, DB, FUSE, RS, true>(a);
+            k_fast_body<A, KIND, INV, T, LPB, DB, FUSE, RS, true>(a, blockIdx.x);
             return;
         }
     }
-    k_fast_body<A, KIND, INV, T, LPB, DB, FUSE, RS, false>(a);
+    k_fast_body<A, KIND, INV, T, LPB, DB, FUSE, RS, false>(a, blockIdx.x);
 }
 
 }  // namespace kfa

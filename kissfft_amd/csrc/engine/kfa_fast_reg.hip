@@ -47,6 +47,26 @@ struct TwPad<Q15P> {
     static constexpr int fmt = 1;
 };
 
+// the arithmetic's name in kernel source (for fused launches compiled at run time)
+template <class A>
+struct ArithName;
+template <>
+struct ArithName<F32> {
+    static constexpr const char *value = "kfa::F32";
+};
+template <>
+struct ArithName<Q15> {
+    static constexpr const char *value = "kfa::Q15";
+};
+template <>
+struct ArithName<Q15P> {
+    static constexpr const char *value = "kfa::Q15P";
+};
+template <>
+struct ArithName<Q31> {
+    static constexpr const char *value = "kfa::Q31";
+};
+
 template <class A, int KIND, int T, int LPB, bool DB, bool FUSE, int... R>
 static void add_variant(std::vector<FastVariant> &v, int type, const char *name)
 {
@@ -71,6 +91,9 @@ static void add_variant(std::vector<FastVariant> &v, int type, const char *name)
         f.tw_fmt = TwPad<A>::fmt;
         f.tw_len = RS::tw_len();
         f.name = name;
+        f.arith = ArithName<A>::value;
+        f.db = DB;
+        f.fuse = FUSE;
         f.launch = inv ? &launch_fast<A, KIND, true, T, LPB, DB, FUSE, RS>
                        : &launch_fast<A, KIND, false, T, LPB, DB, FUSE, RS>;
         v.push_back(f);

@@ -25,6 +25,11 @@ struct FastVariant {
     const char *name;
     int (*launch)(const FastArgs &, hipStream_t);
     const void *rtc = nullptr;  // runtime-compiled variant (kfa_rtc.hip): launched through it
+    // k_fast variants (both built-in and runtime-compiled): the template arguments a fused
+    // launch needs to name the same kernel body; arith == nullptr: another kernel
+    const char *arith = nullptr;
+    bool db = false;
+    bool fuse = false;
 };
 
 const std::vector<FastVariant> &fast_registry();
@@ -71,7 +76,7 @@ struct BlueArgs;
 // Bluestein variants (kfa_blue.hpp): one convolution length M, float.
 struct BlueVariant {
     int M;
-    int T, LPB;
+    int T, LPB;  // (k_blue<T, LPB, Radices<tw_pad, radices...>>)
     std::vector<int> radices;
     int tw_pad;
     size_t lds;

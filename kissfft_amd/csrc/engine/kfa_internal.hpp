@@ -198,6 +198,21 @@ int blue_prepare(DevPlan &P, bool *applies, std::shared_ptr<const BlueChoice> *k
 // one Bluestein launch for plan P; b carries buffers, segments, counts and distances
 struct BlueArgs;
 int blue_launch(DevPlan &P, BlueArgs &b, hipStream_t s);
+// fills b's tables from choice bc (plan length n) and returns the variant that runs it
+struct BlueVariant;
+const BlueVariant *blue_fill(const BlueChoice &bc, int n, BlueArgs &b);
+
+// One member of a fused launch: a k_fast body (fv, a variant with `arith` set) or a
+// Bluestein body (bv).
+struct FastVariant;
+struct FusedMember {
+    const FastVariant *fv;
+    const BlueVariant *bv;
+};
+// The runtime-compiled kernel that runs members m[0..] in one grid of NT-thread workgroups
+// (member i on the workgroups of FusedTab entry i); 0 and *fn on success, nonzero when
+// the runtime compiler is unavailable or the kernel would spill.
+int rtc_fused(const std::vector<FusedMember> &m, int NT, hipFunction_t *fn);
 
 // Tuning (first-use timing of variants) runs on a private stream of the device, after the
 // work queued on the caller's stream, never on that stream itself, and not at all while the

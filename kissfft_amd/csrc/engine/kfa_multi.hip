@@ -27,7 +27,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -66,6 +68,7 @@ struct Multi {
         const FastVariant *v;
         FastArgs a;
         size_t bytes;
+        bool fused;                      // runs inside the fused launch
     };
     std::vector<Group> groups;
     int group_items = 0;
@@ -73,9 +76,18 @@ struct Multi {
         std::shared_ptr<DevPlan> P;
         std::shared_ptr<const BlueChoice> bc;  // the tables the launches (and a graph) use
         BlueArgs a;
+        bool fused;
     };
     std::vector<BlueGroup> blue;
     int blue_items = 0;
+    struct Fused {                       // groups of one workgroup size as one grid
+        hipFunction_t fn = nullptr;
+        FusedTab tab{};
+        long long blocks = 0;
+        int NT = 0, members = 0;
+        size_t lds = 0;
+        DevBuf args;                     // the members' argument blocks
+    } fused;
     std::vector<hipStream_t> helpers;    // fork / join streams (default priority)
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -107,6 +119,118 @@ static int run_long(const Multi::Long &L, hipStream_t s)
     return run_c2c(*L.P, L.in, 1, nc, L.out, 1, nc, L.howmany, s);
 }
 
+// Puts the compiled-kernel groups and Bluestein groups that share the most common
+// workgroup size (by bytes moved, at least two of them) into one fused launch: a kernel
+// compiled at run time from those groups' own bodies (rtc_fused), one grid whose workgroup
+// ranges run the groups in order of size, largest first, so the small groups fill the
+// large ones' tail instead of each draining the device on its own.  Groups of other sizes,
+// and every group when the runtime compiler is unavailable, keep their own launches.
+// KISSFFT_AMD_MULTI_FUSED=0 turns fusing off.
+static void fuse_groups(Multi &M)
+{
+    if (!env_flag("KISSFFT_AMD_MULTI_FUSED", true))
+        return;
+    struct Cand {
+        int g;
+        bool blue;
+        int NT;
+        size_t lds, work;
+        long long blocks;
+        FusedMember m;
+    };
+    std::vector<Cand> c;
+    constexpr size_t kMaxLds = 64 * 1024;  // dynamic LDS of a module kernel without attributes
+    for (size_t i = 0; i < M.groups.size(); ++i) {
+        const FastVariant *v = M.groups[i].v;
+        if (!v->arith)
+            continue;  // another kernel than k_fast
+        const size_t lds = (v->db ? 2 : 1) * (size_t)v->LPB * v->n * (16 / v->tw_pad);
+        if (lds > kMaxLds)
+            continue;
+        const long long blocks = (M.groups[i].a.nlines + v->LPB - 1) / v->LPB;
+        c.push_back({(int)i, false, v->T * v->LPB, lds, M.groups[i].bytes, blocks, {v, nullptr}});
+    }
+    for (size_t i = 0; i < M.blue.size(); ++i) {
+        Multi::BlueGroup &G = M.blue[i];
+        const BlueVariant *bv = blue_fill(*G.bc, G.P->n, G.a);
+        if (!bv || bv->lds > kMaxLds)
+            continue;
+        const long long blocks = (G.a.nlines + bv->LPB - 1) / bv->LPB;
+        // work: the two convolution transforms of M points per line
+        const size_t work = (size_t)G.a.nlines * bv->M * sizeof(float2) * 2;
+        c.push_back({(int)i, true, bv->T * bv->LPB, bv->lds, work, blocks, {nullptr, bv}});
+    }
+    std::map<int, std::pair<int, size_t>> by_nt;  // NT -> (groups, work)
+    for (const Cand &x : c) {
+        by_nt[x.NT].first++;
+        by_nt[x.NT].second += x.work;
+    }
+    int NT = 0;
+    size_t best = 0;
+    for (const auto &kv : by_nt)
+        if (kv.second.first >= 2 && kv.second.second > best) {
+            best = kv.second.second;
+            NT = kv.first;
+        }
+    if (!NT)
+        return;
+    std::vector<Cand> mem;
+    for (const Cand &x : c)
+        if (x.NT == NT && mem.size() < KFA_MAX_FUSED)
+            mem.push_back(x);
+    std::stable_sort(mem.begin(), mem.end(), [](const Cand &x, const Cand &y) { return x.work > y.work; });
+    std::vector<FusedMember> fm;
+    for (const Cand &x : mem)
+        fm.push_back(x.m);
+    hipFunction_t fn = nullptr;
+    if (rtc_fused(fm, NT, &fn)) {
+        (void)hipGetLastError();
+        return;
+    }
+    // the members' argument blocks, each at a 256-byte boundary
+    std::vector<size_t> off;
+    size_t total = 0;
+    for (const Cand &x : mem) {
+        off.push_back(total);
+        total += ((x.blue ? sizeof(BlueArgs) : sizeof(FastArgs)) + 255) / 256 * 256;
+    }
+    std::vector<unsigned char> h(total, 0);
+    for (size_t i = 0; i < mem.size(); ++i) {
+        if (mem[i].blue)
+            memcpy(h.data() + off[i], &M.blue[mem[i].g].a, sizeof(BlueArgs));
+        else
+            memcpy(h.data() + off[i], &M.groups[mem[i].g].a, sizeof(FastArgs));
+    }
+    Multi::Fused F;
+    if (dev_upload(h.data(), total, F.args)) {
+        (void)hipGetLastError();
+        return;
+    }
+    long long end = 0;
+    for (size_t i = 0; i < mem.size(); ++i) {
+        F.tab.arg[i] = static_cast<const unsigned char *>(F.args.get()) + off[i];
+        end += mem[i].blocks;
+        F.tab.end[i] = end;
+        F.lds = std::max(F.lds, mem[i].lds);
+    }
+    if (end <= 0 || end > 0x7fffffffLL)
+        return;
+    F.fn = fn;
+    F.blocks = end;
+    F.NT = NT;
+    F.members = (int)mem.size();
+    for (const Cand &x : mem) {
+        if (x.blue)
+            M.blue[x.g].fused = true;
+        else
+            M.groups[x.g].fused = true;
+    }
+    M.fused = std::move(F);
+    if (env_flag("KISSFFT_AMD_VERBOSE", false))
+        fprintf(stderr, "[kissfft_amd] multi-batch: %d groups fused, %d threads, %lld workgroups\n", M.fused.members,
+                NT, end);
+}
+
 // Every launch of the multi-batch on s (forked over s and the helper streams and joined
 // on s when there are helpers).
 static int run_multi(Multi &M, hipStream_t s)
@@ -134,14 +258,26 @@ static int run_multi(Multi &M, hipStream_t s)
     }
     size_t next = 0;
     auto pick = [&]() { return st[next++ % st.size()]; };
+    if (M.fused.fn) {
+        ++g_launches;
+        FusedTab tab = M.fused.tab;
+        void *params[] = {&tab};
+        const hipError_t e = hipModuleLaunchKernel(M.fused.fn, (unsigned)M.fused.blocks, 1, 1, (unsigned)M.fused.NT, 1,
+                                                   1, (unsigned)M.fused.lds, pick(), params, nullptr);
+        if (e != hipSuccess)
+            rc = fail(-3, "fused multi-batch launch failed: %s", hipGetErrorString(e));
+    }
     // groups are sorted largest first at creation
     for (size_t i = 0; i < M.groups.size() && rc == 0; ++i) {
+        if (M.groups[i].fused)
+            continue;
         ++g_launches;
         if (fast_launch(*M.groups[i].v, M.groups[i].a, pick()))
             rc = fail(-3, "fast kernel launch failed");
     }
     for (size_t i = 0; i < M.blue.size() && rc == 0; ++i)
-        rc = blue_launch(*M.blue[i].P, M.blue[i].a, pick());
+        if (!M.blue[i].fused)
+            rc = blue_launch(*M.blue[i].P, M.blue[i].a, pick());
     if (rc == 0 && M.ntiles > 0) {
         hipStream_t sp = pick();
         switch (M.type) {
@@ -362,6 +498,7 @@ void *kfa_multi_create(int type, const kfa_multi_item *items, int nitems)
     }
     std::stable_sort(M->groups.begin(), M->groups.end(),
                      [](const Multi::Group &x, const Multi::Group &y) { return x.bytes > y.bytes; });
+    fuse_groups(*M);
     int nhelpers = kMultiHelpers;
     if (const char *e = getenv("KISSFFT_AMD_MULTI_HELPERS"))
         nhelpers = std::max(0, std::min(atoi(e), 8));
@@ -481,6 +618,21 @@ int kfa_multi_layout(void *m, int *groups, int *group_items, int *bluestein_item
         *bluestein_items = M->blue_items;
     if (multipass_items)
         *multipass_items = (int)M->longs.size();
+    return 0;
+}
+
+int kfa_multi_fused(void *m, int *members, long long *workgroups, int *threads)
+{
+    Multi *M = static_cast<Multi *>(m);
+    if (!M)
+        return -2;
+    const bool f = M->fused.fn != nullptr;
+    if (members)
+        *members = f ? M->fused.members : 0;
+    if (workgroups)
+        *workgroups = f ? M->fused.blocks : 0;
+    if (threads)
+        *threads = f ? M->fused.NT : 0;
     return 0;
 }
 

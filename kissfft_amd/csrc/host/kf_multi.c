@@ -105,6 +105,11 @@ int kiss_fft_multi_layout(kiss_fft_multi m, int *groups, int *group_items, int *
     return m ? kfa_multi_layout(m->engine, groups, group_items, bluestein_items, multipass_items) : -2;
 }
 
+int kiss_fft_multi_fused(kiss_fft_multi m, int *members, long long *workgroups, int *threads)
+{
+    return m ? kfa_multi_fused(m->engine, members, workgroups, threads) : -2;
+}
+
 void kiss_fft_multi_free(kiss_fft_multi m)
 {
     if (!m)

@@ -127,6 +127,7 @@ KFA_HIDDEN int kfa_multi_replay(void *m, void *stream, int sync);
 KFA_HIDDEN int kfa_multi_info(void *m, long long *tiles, int *grid, int *persistent_items);
 KFA_HIDDEN int kfa_multi_layout(void *m, int *groups, int *group_items, int *bluestein_items,
                                 int *multipass_items);
+KFA_HIDDEN int kfa_multi_fused(void *m, int *members, long long *workgroups, int *threads);
 KFA_HIDDEN void kfa_multi_destroy(void *m);
 
 /* Fast FIR by overlap-scrap: nblocks windows in[k*ngood .. +nfft) -> out[k*ngood .. +ngood).

@@ -180,6 +180,72 @@ def test_many_small_batches_grouped():
     m.close()
 
 
+def test_fused_launch_same_results(monkeypatch):
+    """The compiled-kernel and Bluestein groups of the most common workgroup size run as
+    one fused launch (a kernel compiled at run time from their own bodies): the same
+    results as separate launches (KISSFFT_AMD_MULTI_FUSED=0) -- bit for bit in Q15 --
+    within the float bar of FP64, and every graph replay equals the eager result."""
+    import torch
+    dev = torch.device("cuda:0")
+    menu = [(C2C, 256), (C2C, 1000), (C2C, 1009), (C2C, 1200), (C2C_INV, 3072), (R2C, 4096), (C2R, 4000),
+            (C2C, 120)]
+    rng = np.random.default_rng(14)
+    for dtype in ("float", "int16_t"):
+        lib = kissfft_amd.load(dtype)
+        items = [(menu[i % 8][0], menu[i % 8][1], int(rng.integers(1, 70))) for i in range(40)]
+        if dtype == "int16_t":  # Q15: no Bluestein; 1009 takes the persistent launch
+            items = [it for it in items if it[1] != 1009]
+        bufs = [make(torch, dev, k, n, b, 800 + i, dtype) for i, (k, n, b) in enumerate(items)]
+        spec = [(k, n, x, y, b) for (k, n, b), (x, y) in zip(items, bufs)]
+        monkeypatch.setenv("KISSFFT_AMD_MULTI_FUSED", "0")
+        sep = lib.multi(spec)
+        assert sep.info()["fused_groups"] == 0
+        sep.exec()
+        want = [y.clone() for _, y in bufs]
+        sep.close()
+        monkeypatch.delenv("KISSFFT_AMD_MULTI_FUSED")
+        fz = lib.multi(spec)
+        info = fz.info()
+        say("fused", dtype, info)
+        if dtype == "float":
+            assert info["fused_groups"] >= 2 and info["fused_workgroups"] > 0 and info["fused_threads"] > 0
+        for _, y in bufs:
+            y.fill_(7)
+        torch.cuda.synchronize()
+        fz.exec()
+        eager = [y.clone() for _, y in bufs]
+        O = Oracle(dtype) if dtype != "float" else None
+        for (k, n, b), (x, y), w in zip(items, bufs, want):
+            say("fused item", dtype, k, n, b)
+            if O is not None:
+                assert torch.equal(y, w), (k, n, b)
+                xs = x.cpu().numpy()
+                ref = {C2C: lambda: O.fft(xs), C2C_INV: lambda: O.fft(xs, inverse=True), R2C: lambda: O.fftr(xs),
+                       C2R: lambda: O.fftri(xs, n)}[k]()
+                np.testing.assert_array_equal(y.cpu().numpy(), ref, err_msg=f"q15 kind {k} n {n}")
+            else:
+                got = y.cpu().numpy()
+                got = got if k == C2R else cplx(got)
+                assert rel_l2(got, fp64_ref(k, n, x.cpu().numpy())) <= ABS_FLOAT_BOUND, (k, n, b)
+                w = w.cpu().numpy()
+                w = w if k == C2R else cplx(w)
+                assert rel_l2(got, w) <= ABS_FLOAT_BOUND, (k, n, b)
+        st = torch.cuda.Stream()
+        fz.capture(st)
+        for rep in range(3):
+            with torch.cuda.stream(st):
+                for _, y in bufs:
+                    y.fill_(-3)
+            fz.replay(st)
+            st.synchronize()
+            for (k, n, b), (_, y), e in zip(items, bufs, eager):
+                assert torch.equal(y, e), (dtype, rep, k, n, b)
+            torch.cuda.synchronize()
+            fz.exec()  # eager work between the replays
+            torch.cuda.synchronize()
+        fz.close()
+
+
 def test_helper_streams_same_results(monkeypatch):
     """KISSFFT_AMD_MULTI_HELPERS=n forks the launches over n streams of the multi-batch and
     joins them on the caller's stream: eager execution and graph replays give the same bits
