@@ -311,11 +311,26 @@ def other_configs(lib, stream, dev) -> dict:
     mb.capture(stream)
     t_multi = _time(lambda: mb.replay(stream), stream, 20)
 
+    # the same multi-batch with one launch per group instead of the fused launch
+    prev = os.environ.get("KISSFFT_AMD_MULTI_FUSED")
+    os.environ["KISSFFT_AMD_MULTI_FUSED"] = "0"
+    try:
+        mb_sep = kissfft_amd.MultiBatch(lib, items)
+    finally:
+        if prev is None:
+            del os.environ["KISSFFT_AMD_MULTI_FUSED"]
+        else:
+            os.environ["KISSFFT_AMD_MULTI_FUSED"] = prev
+    mb_sep.capture(stream)
+    t_sep = _time(lambda: mb_sep.replay(stream), stream, 20)
+    mb_sep.close()
+
     def per_batch():
         for c in calls:
             c()
     t_calls = _time(per_batch, stream, 20)
     rec("multi_uneven_64_batches", t_multi, mbytes, mtrans,
+        separate_launches_us=round(t_sep * 1e6, 1),
         per_batch_calls_us=round(t_calls * 1e6, 1), speedup_vs_per_batch_calls=round(t_calls / t_multi, 2),
         kinds_lengths="c2c 256/1000/1009/1200/120, c2c inverse 3072, r2c 4096, c2r 4000",
         launch=mb.info())

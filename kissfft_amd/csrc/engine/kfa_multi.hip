@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -88,6 +89,7 @@ struct Multi {
         size_t lds = 0;
         DevBuf args;                     // the members' argument blocks
     } fused;
+    std::vector<std::unique_ptr<FastVariant>> shaped;  // reshaped members of the fused launch
     std::vector<hipStream_t> helpers;    // fork / join streams (default priority)
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -119,13 +121,52 @@ static int run_long(const Multi::Long &L, hipStream_t s)
     return run_c2c(*L.P, L.in, 1, nc, L.out, 1, nc, L.howmany, s);
 }
 
-// Puts the compiled-kernel groups and Bluestein groups that share the most common
-// workgroup size (by bytes moved, at least two of them) into one fused launch: a kernel
-// compiled at run time from those groups' own bodies (rtc_fused), one grid whose workgroup
-// ranges run the groups in order of size, largest first, so the small groups fill the
-// large ones' tail instead of each draining the device on its own.  Groups of other sizes,
-// and every group when the runtime compiler is unavailable, keep their own launches.
-// KISSFFT_AMD_MULTI_FUSED=0 turns fusing off.
+// The row kernel of variant v with another thread shape: NT threads per workgroup as
+// T x LPB lines, T chosen to keep v's points per thread as nearly as NT allows.  The
+// radix schedule, the stage tables and every butterfly stay v's (the thread count only
+// deals the butterflies of each stage): fixed-point results are v's bit for bit, float
+// results may round differently in the last bit where the compiler forms multiply-adds
+// differently for the new shape.  Variants with the real split or merge fused into a stage
+// (which ties T to that stage) do not reshape.
+static bool reshape(const FastVariant &v, int NT, FastVariant &out, size_t &lds)
+{
+    if (!v.arith || v.fuse || NT <= 0)
+        return false;
+    const int pmin = *std::min_element(v.radices.begin(), v.radices.end());
+    const double want = std::log((double)v.n / v.T);
+    bool found = false;
+    double err = 0;
+    for (int lpb = 1; lpb <= NT; ++lpb) {
+        if (NT % lpb)
+            continue;
+        const int T = NT / lpb;
+        if (T > v.n / pmin)  // every thread owns a butterfly of the widest stage
+            continue;
+        const size_t l = (v.db ? 2 : 1) * (size_t)lpb * v.n * (16 / v.tw_pad);
+        const double e = std::fabs(std::log((double)v.n / T) - want);
+        if (!found || e < err - 1e-9 || (e < err + 1e-9 && l < lds)) {
+            found = true;
+            err = e;
+            lds = l;
+            out = v;
+            out.T = T;
+            out.LPB = lpb;
+            out.launch = nullptr;
+            out.rtc = nullptr;
+        }
+    }
+    return found;
+}
+
+// Puts the compiled-kernel groups and Bluestein groups of one workgroup size into one
+// fused launch: a kernel compiled at run time from those groups' own bodies (rtc_fused),
+// one grid whose workgroup ranges run the groups in order of size, largest first, so the
+// small groups fill the large ones' tail instead of each draining the device on its own.
+// Row-kernel groups of another size join reshaped to that size (reshape above); the size
+// is the one whose members carry the most work.  Groups that cannot join (Bluestein of
+// another size, fused real split / merge variants of another size, more than 64 KiB of
+// LDS), and every group when the runtime compiler is unavailable, keep their own launches.
+// KISSFFT_AMD_MULTI_FUSED=0 turns fusing off, KISSFFT_AMD_MULTI_RESHAPE=0 reshaping.
 static void fuse_groups(Multi &M)
 {
     if (!env_flag("KISSFFT_AMD_MULTI_FUSED", true))
@@ -160,24 +201,53 @@ static void fuse_groups(Multi &M)
         const size_t work = (size_t)G.a.nlines * bv->M * sizeof(float2) * 2;
         c.push_back({(int)i, true, bv->T * bv->LPB, bv->lds, work, blocks, {nullptr, bv}});
     }
-    std::map<int, std::pair<int, size_t>> by_nt;  // NT -> (groups, work)
-    for (const Cand &x : c) {
-        by_nt[x.NT].first++;
-        by_nt[x.NT].second += x.work;
-    }
+    // the workgroup size whose members (groups of that size, and row-kernel groups that
+    // reshape to it) carry the most work, at least two members
+    const bool reshape_on = env_flag("KISSFFT_AMD_MULTI_RESHAPE", true);
+    auto member = [&](const Cand &x, int NT, Cand &out) {
+        out = x;
+        if (x.NT == NT)
+            return true;
+        if (x.blue || !reshape_on)
+            return false;
+        FastVariant r;
+        if (!reshape(*x.m.fv, NT, r, out.lds) || out.lds > kMaxLds)
+            return false;
+        out.NT = NT;
+        out.blocks = (M.groups[x.g].a.nlines + r.LPB - 1) / r.LPB;
+        M.shaped.push_back(std::make_unique<FastVariant>(r));
+        out.m.fv = M.shaped.back().get();
+        return true;
+    };
     int NT = 0;
     size_t best = 0;
-    for (const auto &kv : by_nt)
-        if (kv.second.first >= 2 && kv.second.second > best) {
-            best = kv.second.second;
-            NT = kv.first;
+    std::vector<int> sizes;
+    for (const Cand &x : c)
+        if (std::find(sizes.begin(), sizes.end(), x.NT) == sizes.end())
+            sizes.push_back(x.NT);
+    for (int nt : sizes) {
+        int count = 0;
+        size_t work = 0;
+        Cand y;
+        for (const Cand &x : c)
+            if (member(x, nt, y)) {
+                ++count;
+                work += x.work;
+            }
+        if (count >= 2 && work > best) {
+            best = work;
+            NT = nt;
         }
+    }
+    M.shaped.clear();
     if (!NT)
         return;
     std::vector<Cand> mem;
-    for (const Cand &x : c)
-        if (x.NT == NT && mem.size() < KFA_MAX_FUSED)
-            mem.push_back(x);
+    for (const Cand &x : c) {
+        Cand y;
+        if (mem.size() < KFA_MAX_FUSED && member(x, NT, y))
+            mem.push_back(y);
+    }
     std::stable_sort(mem.begin(), mem.end(), [](const Cand &x, const Cand &y) { return x.work > y.work; });
     std::vector<FusedMember> fm;
     for (const Cand &x : mem)
