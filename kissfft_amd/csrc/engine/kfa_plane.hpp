@@ -66,13 +66,15 @@ __device__ __forceinline__ void plane_stages(std::integer_sequence<int, S...>, F
     (plane_stage<INV, T, RS, ES, S>(lb, stw, t, active), ...);
 }
 
-// Persistent: workgroup g takes planes g, g + G, g + 2G, ... (G = gridDim.x).  The next
-// plane's loads are issued into registers as soon as the current plane is in LDS, so
-// they are in flight while the current plane is transformed; the stage tables are copied
-// to LDS once, so that the stages issue no global loads (a global load's wait would also
-// wait for the prefetch).
+// Persistent: workgroup g takes planes g, g + G, g + 2G, ... (G = gridDim.x).  PF: the
+// next plane's loads are issued into registers as soon as the current plane is in LDS, so
+// they are in flight while the current plane is transformed (for planes that leave room
+// for one workgroup per CU); without PF a plane is loaded when its turn comes, which saves
+// the registers of the prefetch so that two workgroups fit a CU and overlap each other.
+// The stage tables are copied to LDS once, so that the stages issue no global loads (a
+// global load's wait would also wait for the prefetch).
 // SLAB: planes addressed through rs / nsub / po / pi (else contiguous A x B planes, p * A * B)
-template <bool INV, int TA, int TB, int NTH, class RA, class RB, bool SLAB>
+template <bool INV, int TA, int TB, int NTH, class RA, class RB, bool SLAB, bool PF>
 __global__ void __launch_bounds__(NTH) k_plane(PlaneArgs a)
 {
     typedef F32::store2 St;
@@ -109,9 +111,11 @@ __global__ void __launch_bounds__(NTH) k_plane(PlaneArgs a)
                 v[k] = src[at(e)];
         }
     };
-    if (pl < a.planes)
+    if (PF && pl < a.planes)
         load(pl);
     for (; pl < a.planes; pl += gridDim.x) {
+        if (!PF)
+            load(pl);
 #pragma unroll
         for (int k = 0; k < IT; ++k) {
             const int e = tid + k * NTH;
@@ -119,7 +123,7 @@ __global__ void __launch_bounds__(NTH) k_plane(PlaneArgs a)
                 buf[(e / B) * RSX + e % B] = v[k];
         }
         __syncthreads();
-        if (pl + gridDim.x < a.planes)
+        if (PF && pl + gridDim.x < a.planes)
             load(pl + gridDim.x);  // in flight during the stages below
         {   // rows: B points, TB threads each
             const int line = tid / TB;

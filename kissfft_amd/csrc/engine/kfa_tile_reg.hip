@@ -158,10 +158,10 @@ constexpr size_t plane_lds()
 // fit on the device, each looping over its share of the planes, so that loads of the next
 // plane overlap the transform of the current one); smaller planes run one per workgroup,
 // several workgroups per CU overlapping each other.
-template <bool INV, int TA, int TB, int NTH, class RA, class RB, bool SLAB>
+template <bool INV, int TA, int TB, int NTH, class RA, class RB, bool SLAB, bool PF>
 static int launch_plane_as(const PlaneArgs &a, hipStream_t s)
 {
-    const void *fn = (const void *)k_plane<INV, TA, TB, NTH, RA, RB, SLAB>;
+    const void *fn = (const void *)k_plane<INV, TA, TB, NTH, RA, RB, SLAB, PF>;
     constexpr size_t lds = plane_lds<RA, RB>();
     static long long resident = 0;  // 0: one workgroup per plane
     static std::once_flag once;
@@ -178,20 +178,20 @@ static int launch_plane_as(const PlaneArgs &a, hipStream_t s)
     const long long grid = (resident > 0 && a.planes > resident) ? resident : a.planes;
     if (grid > 0x7fffffffLL)
         return -2;
-    hipLaunchKernelGGL((k_plane<INV, TA, TB, NTH, RA, RB, SLAB>), dim3((unsigned)grid), dim3(NTH), lds, s, a);
+    hipLaunchKernelGGL((k_plane<INV, TA, TB, NTH, RA, RB, SLAB, PF>), dim3((unsigned)grid), dim3(NTH), lds, s, a);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
 // contiguous planes through the plain instance; slabs through the slab instance when the
 // variant has one (else -2)
-template <bool INV, int TA, int TB, int NTH, class RA, class RB, bool SLAB>
+template <bool INV, int TA, int TB, int NTH, class RA, class RB, bool SLAB, bool PF>
 static int launch_plane(const PlaneArgs &a, hipStream_t s)
 {
     constexpr int A = RA::N(), B = RB::N();
     if (a.nsub == 1 && a.rs == B && a.po == (long long)A * B)
-        return launch_plane_as<INV, TA, TB, NTH, RA, RB, false>(a, s);
+        return launch_plane_as<INV, TA, TB, NTH, RA, RB, false, PF>(a, s);
     if constexpr (SLAB)
-        return launch_plane_as<INV, TA, TB, NTH, RA, RB, true>(a, s);
+        return launch_plane_as<INV, TA, TB, NTH, RA, RB, true, PF>(a, s);
     return -2;
 }
 
@@ -201,7 +201,7 @@ static std::vector<int> radix_list()
     return std::vector<int>(RS::r, RS::r + RS::K);
 }
 
-template <int TA, int TB, int NTH, class RA, class RB, bool SLAB = true>
+template <int TA, int TB, int NTH, class RA, class RB, bool SLAB = true, bool PF = true>
 static void add_plane(std::vector<PlaneVariant> &v, const char *name)
 {
     constexpr int B = RB::N();
@@ -222,7 +222,8 @@ static void add_plane(std::vector<PlaneVariant> &v, const char *name)
         f.lds = lds;
         f.name = name;
         f.slab = SLAB;
-        f.launch = inv ? &launch_plane<true, TA, TB, NTH, RA, RB, SLAB> : &launch_plane<false, TA, TB, NTH, RA, RB, SLAB>;
+        f.launch = inv ? &launch_plane<true, TA, TB, NTH, RA, RB, SLAB, PF>
+                       : &launch_plane<false, TA, TB, NTH, RA, RB, SLAB, PF>;
         v.push_back(f);
     }
 }
@@ -244,6 +245,9 @@ const std::vector<PlaneVariant> &plane_registry()
         // 103-111 us against 109-119 us for the three axis passes (x16, alternating runs)
         add_plane<8, 16, 1024, Radices<2, 8, 8>, Radices<2, 8, 5, 3>>(*v, "plane64x120_t8x16");
         add_plane<4, 8, 512, Radices<2, 8, 8>, Radices<2, 8, 5, 3>>(*v, "plane64x120_t4x8");
+        // without the next plane's prefetch: 121 registers instead of 151, two workgroups
+        // per CU
+        add_plane<4, 8, 512, Radices<2, 8, 8>, Radices<2, 8, 5, 3>, true, false>(*v, "plane64x120_t4x8_np");
         return v;
     }();
     return *reg;

@@ -166,10 +166,13 @@ def test_nd_in_place_single_call():
 def plane_registry():
     src = open(os.path.join(ROOT, "kissfft_amd", "csrc", "engine", "kfa_tile_reg.hip")).read()
     out = []
-    for m in re.finditer(r"add_plane<\d+, \d+, \d+, Radices<2, ([\d, ]+)>, Radices<2, ([\d, ]+)>(, false)?>\(\*v, \"(\w+)\"\)", src):
+    # add_plane<TA, TB, NTH, Radices<...>, Radices<...>[, SLAB[, PF]]>(*v, "name")
+    pat = r"add_plane<\d+, \d+, \d+, Radices<2, ([\d, ]+)>, Radices<2, ([\d, ]+)>((?:, (?:true|false))*)>\(\*v, \"(\w+)\"\)"
+    for m in re.finditer(pat, src):
         A = int(np.prod([int(v) for v in m.group(1).split(",")]))
         B = int(np.prod([int(v) for v in m.group(2).split(",")]))
-        out.append((A, B, m.group(4), m.group(3) is None))
+        flags = [f.strip() for f in m.group(3).split(",") if f.strip()]
+        out.append((A, B, m.group(4), not flags or flags[0] == "true"))
     return out
 
 
