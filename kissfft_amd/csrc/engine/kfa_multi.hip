@@ -16,11 +16,14 @@
 //     are carved from one workspace allocated at creation (sized for the largest item's
 //     chunk), and their lazily built choices (tile variants, four-step tables) are decided
 //     at creation too, so these launches are capturable as well.
+// The compiled-kernel and Bluestein groups are then fused into one launch where the
+// runtime compiler allows (fuse_groups below): one grid of one workgroup size running
+// every group's own kernel body on its range of workgroups.
 // KISSFFT_AMD_MULTI_PERSISTENT=1 puts every single-pass item into the persistent launch.
 // Execution issues the launches back to back on the caller's stream (largest group first);
 // capture records them into one hipGraph.  KISSFFT_AMD_MULTI_HELPERS=n (at most 8) forks
 // them round robin over n helper streams joined on the caller's stream instead: measured
-// on 64 uneven batches (8 launches), the launches did not overlap and every hand-over
+// on 64 uneven batches as 8 separate launches (KISSFFT_AMD_MULTI_FUSED=0), they did not overlap and every hand-over
 // between queues cost 10-30 us (graph replay 163-170 us with three helpers, 109-114 us
 // with one, 78 us with none).  Nothing is allocated and no memset is issued on the
 // capturable path.
