@@ -522,7 +522,8 @@ struct TileCache {
     std::map<std::tuple<int, int, int, int>, std::shared_ptr<const TileChoice>> choice;   // dev, n, inv, mode
     std::map<std::tuple<int, long long, int>, BufPair> fs;                                // dev, N, inv -> hi, lo
     std::map<std::tuple<int, long long, int, int>, BufPair> rs;                           // dev, nc, n, inv
-    std::map<std::tuple<int, int, int, int>, std::shared_ptr<const PlaneChoice>> plane;  // dev, A, B, inv
+    // dev, A, B, inv, slab, small (see plane_choice)
+    std::map<std::tuple<int, int, int, int, int, int>, std::shared_ptr<const PlaneChoice>> plane;
 };
 
 static TileCache &tile_cache()
@@ -992,10 +993,16 @@ int blue_launch(DevPlan &P, BlueArgs &b, hipStream_t s)
     return bc->v->launch(b, s) ? fail(-3, "Bluestein kernel launch failed") : 0;
 }
 
-// The plane variant for the last two axes A x B (float) on the current device, or
-// v == nullptr.  Several candidates are timed once on 512 MiB of planes
-// (KISSFFT_AMD_PLANE=0 turns the plane pass off, KISSFFT_AMD_TILE forces a variant by name).
-static int plane_choice(int A, int B, int inverse, hipStream_t s, std::shared_ptr<const PlaneChoice> &out)
+// The plane variant for A x B planes (float) on the current device, or v == nullptr:
+// contiguous planes of the last two axes, or (slab) an outer axis of length A with the
+// last axis, rows rs values apart and nsub planes per value of the axes before it (the
+// plane geometry of PlaneArgs).  Several candidates are timed once, in the caller's
+// geometry, on the caller's planes up to 512 MiB; the choice is kept per geometry kind
+// and per size class (up to 128 MiB per side, which the 256 MB Infinity Cache holds
+// between passes, or more: the ranking differs between the two).
+// (KISSFFT_AMD_PLANE=0 turns the plane pass off, KISSFFT_AMD_TILE forces a variant by name.)
+static int plane_choice(int A, int B, int inverse, bool slab, int rs, long long nsub, long long planes,
+                        hipStream_t s, std::shared_ptr<const PlaneChoice> &out)
 {
     int dev;
     int rc = current_device(&dev);
@@ -1003,7 +1010,8 @@ static int plane_choice(int A, int B, int inverse, hipStream_t s, std::shared_pt
         return rc;
     auto &C = tile_cache();
     std::lock_guard<std::mutex> g(C.mu);
-    const auto key = std::make_tuple(dev, A, B, inverse);
+    const bool small = 8.0 * A * B * (double)planes <= (double)(128 << 20);
+    const auto key = std::make_tuple(dev, A, B, inverse, (int)slab, (int)small);
     auto it = C.plane.find(key);
     if (it != C.plane.end()) {
         out = it->second;
@@ -1018,7 +1026,8 @@ static int plane_choice(int A, int B, int inverse, hipStream_t s, std::shared_pt
         for (const PlaneVariant &v : plane_registry())
             named |= force && *force && !strcmp(force, v.name) && v.A == A && v.B == B;
         for (const PlaneVariant &v : plane_registry())
-            if (v.A == A && v.B == B && v.inverse == inverse && (!named || !strcmp(force, v.name))) {
+            if (v.A == A && v.B == B && v.inverse == inverse && (!slab || v.slab) &&
+                (!named || !strcmp(force, v.name))) {
                 auto c = std::make_shared<PlaneChoice>();
                 c->v = &v;
                 cand.push_back(c);
@@ -1035,18 +1044,23 @@ static int plane_choice(int A, int B, int inverse, hipStream_t s, std::shared_pt
     size_t best = 0;
     hipStream_t ts;
     if (cand.size() > 1 && env_flag("KISSFFT_AMD_TUNE", true) && !stream_capturing(s) && tune_stream(s, &ts) == 0) {
+        const long long sub = slab ? std::max(1LL, nsub) : 1;
         for (long long values = kTuneValues; values >= (kTuneValues >> 4); values >>= 2) {
-            const long long planes = std::max(1LL, values / ((long long)A * B));
-            const size_t bytes = 8 * (size_t)A * B * (size_t)planes;
+            // whole groups of `sub` planes (a slab's planes tile its buffer only so)
+            const long long want = std::min(planes, std::max(1LL, values / ((long long)A * B)));
+            const long long tp = std::max(1LL, want / sub) * sub;
+            const size_t bytes = 8 * (size_t)A * B * (size_t)tp;
             TuneBuf bin(bytes, ts), bout(bytes, ts);
             if (!bin.p || !bout.p)
                 continue;
             if (fill_tune(bin.p, bytes, KFA_F32, ts))
                 break;
             char what[64];
-            snprintf(what, sizeof what, "plane %dx%d", A, B);
+            snprintf(what, sizeof what, "plane %dx%d%s", A, B, slab ? " slab" : "");
             best = time_candidates(cand.size(), ts, [&](size_t i) {
-                const PlaneArgs pa = plane_args(bin.p, bout.p, cand[i]->stw_a, cand[i]->stw_b, planes, A, B);
+                const PlaneArgs pa =
+                    slab ? PlaneArgs{bin.p, bout.p, cand[i]->stw_a, cand[i]->stw_b, tp, rs, sub, (long long)A * rs, B}
+                         : plane_args(bin.p, bout.p, cand[i]->stw_a, cand[i]->stw_b, tp, A, B);
                 return cand[i]->v->launch(pa, ts);
             }, what);
             break;
@@ -2266,7 +2280,11 @@ int run_nd(const std::vector<std::shared_ptr<DevPlan>> &P, const int *dims, cons
                 rs *= dims[j];
             if ((long long)(dims[k] - 1) * rs + nb >= (1LL << 31))
                 continue;
-            if ((rc = plane_choice(dims[k], nb, P[ndims - 1]->inverse, s, pc)))
+            long long mid = 1;
+            for (int j = k + 1; j < ndims - 1; ++j)
+                mid *= dims[j];
+            const long long planes = howmany * (total / ((long long)dims[k] * nb));
+            if ((rc = plane_choice(dims[k], nb, P[ndims - 1]->inverse, k != ndims - 2, (int)rs, mid, planes, s, pc)))
                 return rc;
             if (pc->v && (k == ndims - 2 || pc->v->slab)) {
                 paired = k;
