@@ -519,7 +519,8 @@ typedef std::pair<DevBuf, DevBuf> BufPair;
 
 struct TileCache {
     std::mutex mu;
-    std::map<std::tuple<int, int, int, int>, std::shared_ptr<const TileChoice>> choice;   // dev, n, inv, mode
+    // dev, n, inv, mode, inner, small (see tile_choice)
+    std::map<std::tuple<int, int, int, int, long long, int>, std::shared_ptr<const TileChoice>> choice;
     std::map<std::tuple<int, long long, int>, BufPair> fs;                                // dev, N, inv -> hi, lo
     std::map<std::tuple<int, long long, int, int>, BufPair> rs;                           // dev, nc, n, inv
     // dev, A, B, inv, slab, small (see plane_choice)
@@ -641,15 +642,22 @@ struct TuneBuf {
 // buffers cannot be had the smaller sizes are tried, then the first candidate is taken.
 constexpr long long kTuneValues = 64LL << 20;
 
-static size_t tune_tiles(int n, int mode, const std::vector<std::shared_ptr<TileChoice>> &cand, hipStream_t s)
+// Times the tile candidates of (n, mode) on up to 512 MiB per side: 1024 lines per slice,
+// or for column passes with a caller's geometry (inner > 0) that geometry, inner lines
+// per slice and at most the caller's `outer` slices.
+static size_t tune_tiles(int n, int mode, const std::vector<std::shared_ptr<TileChoice>> &cand, hipStream_t s,
+                         long long inner = 0, long long outer_max = 0)
 {
     hipStream_t ts;
     if (tune_stream(s, &ts))
         return 0;
-    const long long C = 1024;
+    const bool geo = mode == TILE_COLS && inner > 0 && outer_max > 0;
+    const long long C = geo ? inner : 1024;
     const bool real = mode == TILE_RSPLIT || mode == TILE_RMERGE;
     for (long long values = kTuneValues; values >= (kTuneValues >> 4); values >>= 2) {
-        const long long outer = std::max(1LL, values / ((long long)n * C));
+        long long outer = std::max(1LL, values / ((long long)n * C));
+        if (geo)
+            outer = std::min(outer, outer_max);
         const size_t bytes = 8 * (size_t)(n * C + 1) * (size_t)outer;
         const size_t hib = 8 * (size_t)((long long)n * C / 1024 + 1);
         const size_t rtb = 8 * (size_t)(n + C);
@@ -693,8 +701,12 @@ static size_t tune_tiles(int n, int mode, const std::vector<std::shared_ptr<Tile
 // The tile variant for (n, inverse, mode) on the current device (v == nullptr: none);
 // several candidates are timed once (KISSFFT_AMD_TILE forces one by name,
 // KISSFFT_AMD_TUNE=0 takes the first registered; while the caller's stream `s` is being
-// captured the first registered is taken untimed).
-static int tile_choice(int n, int inverse, int mode, hipStream_t s, std::shared_ptr<const TileChoice> &out)
+// captured the first registered is taken untimed).  A column pass of an N-D axis passes
+// its geometry (inner lines per slice, outer slices): its candidates are timed in that
+// geometry and the choice is kept per inner count and size class (up to 128 MiB per
+// side, or more), since the row pitch and the Infinity Cache change their ranking.
+static int tile_choice(int n, int inverse, int mode, hipStream_t s, std::shared_ptr<const TileChoice> &out,
+                       long long inner = 0, long long outer = 0)
 {
     int dev;
     int rc = current_device(&dev);
@@ -702,7 +714,9 @@ static int tile_choice(int n, int inverse, int mode, hipStream_t s, std::shared_
         return rc;
     auto &C = tile_cache();
     std::lock_guard<std::mutex> g(C.mu);
-    const auto key = std::make_tuple(dev, n, inverse, mode);
+    const bool geo = mode == TILE_COLS && inner > 0 && outer > 0;
+    const int small = geo && 8.0 * n * (double)inner * (double)outer <= (double)(128 << 20);
+    const auto key = std::make_tuple(dev, n, inverse, mode, geo ? inner : 0LL, small);
     auto it = C.choice.find(key);
     if (it != C.choice.end()) {
         out = it->second;
@@ -731,7 +745,7 @@ static int tile_choice(int n, int inverse, int mode, hipStream_t s, std::shared_
     }
     size_t best = 0;
     if (cand.size() > 1 && env_flag("KISSFFT_AMD_TUNE", true) && !stream_capturing(s))
-        best = tune_tiles(n, mode, cand, s);
+        best = geo ? tune_tiles(n, mode, cand, s, inner, outer) : tune_tiles(n, mode, cand, s);
     std::shared_ptr<const TileChoice> tc = cand.empty() ? std::make_shared<const TileChoice>() : cand[best];
     if (tc->v && env_flag("KISSFFT_AMD_VERBOSE", false))
         fprintf(stderr, "[kissfft_amd] tile n=%d inverse=%d mode=%d: %s\n", n, inverse, mode, tc->v->name);
@@ -2212,7 +2226,7 @@ static int run_axis(DevPlan &P, const void *in, void *out, long long outer, long
     }
     if (P.type == KFA_F32) {
         std::shared_ptr<const TileChoice> tc;
-        int rc = tile_choice((int)len, P.inverse, TILE_COLS, s, tc);
+        int rc = tile_choice((int)len, P.inverse, TILE_COLS, s, tc, inner, outer);
         if (rc)
             return rc;
         if (tc->v) {
