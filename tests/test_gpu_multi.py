@@ -35,6 +35,11 @@ def make(torch, dev, kind, n, b, seed, dtype):
         shape_in = {C2C: (b, n, 2), C2C_INV: (b, n, 2), R2C: (b, n), C2R: (b, n // 2 + 1, 2)}[kind]
         x = torch.from_numpy(W.uniform_pm1(seed, int(np.prod(shape_in))).reshape(shape_in)).to(dev)
         tdt = torch.float32
+    elif dtype == "int32_t":
+        shape_in = {C2C: (b, n, 2), C2C_INV: (b, n, 2), R2C: (b, n), C2R: (b, n // 2 + 1, 2)}[kind]
+        q = np.random.default_rng(seed).integers(-(1 << 31), 1 << 31, int(np.prod(shape_in)), dtype=np.int64)
+        x = torch.from_numpy(q.astype(np.int32).reshape(shape_in)).to(dev)
+        tdt = torch.int32
     else:
         shape_in = {C2C: (b, n, 2), C2C_INV: (b, n, 2), R2C: (b, n), C2R: (b, n // 2 + 1, 2)}[kind]
         x = torch.from_numpy(W.q15_uniform(seed, int(np.prod(shape_in))).reshape(shape_in)).to(dev)
@@ -184,7 +189,7 @@ def test_fused_launch_same_results(monkeypatch):
     """The compiled-kernel and Bluestein groups run as one fused launch (a kernel compiled
     at run time from their own bodies, row kernels of other workgroup sizes reshaped to the
     launch's).  Without reshaping the results are bit for bit those of separate launches
-    (KISSFFT_AMD_MULTI_FUSED=0); with it, Q15 stays bit for bit (and equal to the oracle)
+    (KISSFFT_AMD_MULTI_FUSED=0); with it, Q15 and Q31 stay bit for bit (and equal to the oracle)
     and float stays within the float bar of both the separate launches and FP64 (a
     reshaped kernel may round a last bit differently); every graph replay equals the eager
     result."""
@@ -193,10 +198,10 @@ def test_fused_launch_same_results(monkeypatch):
     menu = [(C2C, 256), (C2C, 1000), (C2C, 1009), (C2C, 1200), (C2C_INV, 3072), (R2C, 4096), (C2R, 4000),
             (C2C, 120)]
     rng = np.random.default_rng(14)
-    for dtype in ("float", "int16_t"):
+    for dtype in ("float", "int16_t", "int32_t"):
         lib = kissfft_amd.load(dtype)
         items = [(menu[i % 8][0], menu[i % 8][1], int(rng.integers(1, 70))) for i in range(40)]
-        if dtype == "int16_t":  # Q15: no Bluestein; 1009 takes the persistent launch
+        if dtype != "float":  # fixed point: no Bluestein; 1009 takes the persistent launch
             items = [it for it in items if it[1] != 1009]
         bufs = [make(torch, dev, k, n, b, 800 + i, dtype) for i, (k, n, b) in enumerate(items)]
         spec = [(k, n, x, y, b) for (k, n, b), (x, y) in zip(items, bufs)]
