@@ -170,8 +170,84 @@ __device__ __forceinline__ Cf cf_rot(Cf a)
     return INV ? Cf{-a.i, a.r} : Cf{a.i, -a.r};
 }
 
+// Compile-time cos / sin of 2*pi*j/p in double, rounded once to float: Taylor series
+// after reducing the angle to [-pi, pi] (30 terms: the series' remainder there is far
+// below double precision).
+constexpr double kfa_pi = 3.14159265358979323846264338327950288;
+constexpr double taylor_cos(double x)
+{
+    double t = 1, s = 1;
+    for (int k = 1; k < 30; ++k) {
+        t *= -x * x / ((2.0 * k - 1) * (2.0 * k));
+        s += t;
+    }
+    return s;
+}
+constexpr double taylor_sin(double x)
+{
+    double t = x, s = x;
+    for (int k = 1; k < 30; ++k) {
+        t *= -x * x / ((2.0 * k) * (2.0 * k + 1));
+        s += t;
+    }
+    return s;
+}
+constexpr double angle_2pi(int j, int p)
+{
+    const double x = 2 * kfa_pi * (j % p) / p;
+    return x > kfa_pi ? x - 2 * kfa_pi : x;
+}
+template <int P>
+struct Roots {
+    float c[P], s[P];  // cos / sin of 2*pi*j/P
+};
+template <int P>
+constexpr Roots<P> make_roots()
+{
+    Roots<P> r{};
+    for (int j = 0; j < P; ++j) {
+        r.c[j] = (float)taylor_cos(angle_2pi(j, P));
+        r.s[j] = (float)taylor_sin(angle_2pi(j, P));
+    }
+    return r;
+}
+
+// Any odd radix (7, 11, 13, ...; the schedules of lengths with such factors, kf_factor's
+// generic radices, kiss_fft.c:191-233): the pairs j, p - j share a cosine and an opposite
+// sine, so X[k] = x0 + sum_j c(jk) (x_j + x_{p-j}) -/+ i sum_j s(jk) (x_j - x_{p-j}) and
+// X[p-k] takes the other sign: (p-1)^2 / 2 real products per component.
 template <int P, bool INV>
-struct Dft;
+struct Dft {
+    static_assert(P % 2 == 1 && P >= 7, "generic radix: odd, 7 or more");
+    __device__ __forceinline__ static void run(Cf *v)
+    {
+        constexpr int H = (P - 1) / 2;
+        constexpr Roots<P> R = make_roots<P>();
+        Cf t[H], d[H];
+        const Cf x0 = v[0];
+        Cf s0 = x0;
+#pragma unroll
+        for (int j = 1; j <= H; ++j) {
+            t[j - 1] = cf_add(v[j], v[P - j]);
+            d[j - 1] = cf_sub(v[j], v[P - j]);
+            s0 = cf_add(s0, t[j - 1]);
+        }
+#pragma unroll
+        for (int k = 1; k <= H; ++k) {
+            Cf a = x0, b{0.f, 0.f};
+#pragma unroll
+            for (int j = 1; j <= H; ++j) {
+                const float c = R.c[(j * k) % P], sn = R.s[(j * k) % P];
+                a = Cf{a.r + c * t[j - 1].r, a.i + c * t[j - 1].i};
+                b = Cf{b.r + sn * d[j - 1].r, b.i + sn * d[j - 1].i};
+            }
+            const Cf rb = cf_rot<INV>(b);
+            v[k] = cf_add(a, rb);
+            v[P - k] = cf_sub(a, rb);
+        }
+        v[0] = s0;
+    }
+};
 
 template <bool INV>
 struct Dft<2, INV> {

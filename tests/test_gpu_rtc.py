@@ -1,9 +1,9 @@
 """GPU tests of runtime-compiled row kernels (kfa_rtc.hip) for lengths outside the
 built-in registry, and of the persisted table of tuned choices.
 
-Lengths whose radices are 2, 3, 4, 5 and whose line fits one LDS line get the compile-time
-Stockham kernel k_fast compiled on first use (fixed point with the reference schedule of
-kf_factor, kiss_fft.c:306-328).  Results: bit-exact against the oracle for Q15 / Q31,
+Lengths whose radices are 2, 3, 4, 5 (float: also 7, 11, 13) and whose line fits one LDS
+line get the compile-time Stockham kernel k_fast compiled on first use (fixed point with
+the reference schedule of kf_factor, kiss_fft.c:306-328).  Results: bit-exact against the oracle for Q15 / Q31,
 within the float bar of FP64 for float; the engine's own report (KISSFFT_AMD_VERBOSE=1)
 shows that the runtime-compiled variant ran.
 """
@@ -42,7 +42,9 @@ def rnd(rng, dtype, shape):
 
 
 @pytest.mark.parametrize("dtype,n", [("float", 1536), ("float", 2187), ("float", 3000), ("float", 6561),
-                                     ("float", 7680), ("int16_t", 1536), ("int16_t", 3000), ("int16_t", 6144),
+                                     ("float", 7680), ("float", 2401), ("float", 1001), ("float", 1331),
+                                     ("float", 2197), ("float", 7 * 11 * 13 * 5), ("float", 784),
+                                     ("int16_t", 1536), ("int16_t", 3000), ("int16_t", 6144),
                                      ("int32_t", 1536), ("int32_t", 2187)])
 def test_rtc_c2c(rtc_env, capfd, dtype, n):
     lib = kissfft_amd.load(dtype)
@@ -67,7 +69,8 @@ def test_rtc_c2c(rtc_env, capfd, dtype, n):
     assert any(f.name.startswith("rtc-") for f in rtc_env.iterdir())  # code objects cached
 
 
-@pytest.mark.parametrize("dtype,n", [("float", 3072 * 2), ("float", 6000), ("int16_t", 3000)])
+@pytest.mark.parametrize("dtype,n", [("float", 3072 * 2), ("float", 6000), ("float", 2 * 2401), ("float", 2002),
+                                     ("int16_t", 3000)])
 def test_rtc_real(rtc_env, capfd, dtype, n):
     lib = kissfft_amd.load(dtype)
     lib.L.kiss_fft_cleanup()
