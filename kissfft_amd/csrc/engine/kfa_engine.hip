@@ -481,6 +481,24 @@ int make_stage_table_raw(const unsigned char *h_tw, int type, long long n, const
         fill_to(c0 + rup(2, pad));
         m *= p;
     }
+    // fixed point: the generic stages' combined entries (Radices::gen_off / GR in
+    // kfa_fast.hpp: row u of stage s holds table[q * P * (u + q1 * m) mod n], q1 major)
+    if (type != KFA_F32) {
+        m = 1;
+        for (int p : rad) {
+            const bool gen = !(p == 2 || p == 3 || p == 4 || p == 5 || p == 8 || p == 16);
+            const long long Pst = n / (p * m);
+            if (gen)
+                for (long long u = 0; u < m; ++u) {
+                    const size_t row0 = h.size() / cb;
+                    for (int q1 = 0; q1 < p; ++q1)
+                        for (int q = 1; q < p; ++q)
+                            put((long long)q * Pst * (u + (long long)q1 * m));
+                    fill_to(row0 + rup(p * (p - 1), pad));
+                }
+            m *= p;
+        }
+    }
     RelaxCapture relax;
     return dev_upload(h.data(), h.size(), out);
 }

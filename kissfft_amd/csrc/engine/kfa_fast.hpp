@@ -64,6 +64,24 @@ struct Radices {
         return o;
     }
     static constexpr int tw_len() { return tw_off(K); }
+    // Generic radices (any p but 2, 3, 4, 5, 8, 16).  Fixed point runs them as
+    // kf_bfly_generic does (kiss_fft.c:191-233): output q1 of column u sums
+    // C_MUL(x'_q, table[q * P * (u + q1 * m) mod N]) over q, so after the table above a
+    // fixed-point stage table holds, per generic stage, m rows of p * (p - 1) such
+    // entries (row u: q1 major, q minor), padded to G.
+    static constexpr bool gen(int s)
+    {
+        return !(r[s] == 2 || r[s] == 3 || r[s] == 4 || r[s] == 5 || r[s] == 8 || r[s] == 16);
+    }
+    static constexpr int GR(int s) { return round_up(r[s] * (r[s] - 1), G); }
+    static constexpr int gen_off(int s)
+    {
+        int o = tw_len();
+        for (int i = 0; i < s; ++i)
+            if (gen(i))
+                o += m(i) * GR(i);
+        return o;
+    }
 };
 
 // One launch may cover several separate buffers ("segments", e.g. the requests of a
@@ -546,6 +564,7 @@ struct FastStage {
             const int u = b / P;
             const St *row = stw + TWO + u * PR;
             if constexpr (IsPacked<A>::value) {
+                static_assert(!RS::gen(S), "packed Q15 lines take radices 2, 3, 4, 5 only");
                 // stage table of pre-arranged twiddle word pairs (see kfa_q15p.hpp)
                 const uint2 *tab = reinterpret_cast<const uint2 *>(stw);
                 const uint2 *trow = tab + TWO + u * PR;
@@ -575,6 +594,23 @@ struct FastStage {
 #pragma unroll
                 for (int r = 0; r < p; ++r)
                     v[j][r].w = x[r];
+            } else if constexpr (A::fixed && RS::gen(S)) {
+                // kf_bfly_generic (kiss_fft.c:191-233): every input divided by p, then each
+                // output the first input plus the rounded products of the others with
+                // their table entries, summed in kiss_fft_scalar (C_ADDTO)
+                const St *grow = stw + RS::gen_off(S) + u * RS::GR(S);
+                C x[p];
+#pragma unroll
+                for (int q = 0; q < p; ++q)
+                    x[q] = O::fixdiv(v[j][q], p);
+#pragma unroll
+                for (int q1 = 0; q1 < p; ++q1) {
+                    C acc = x[0];
+#pragma unroll
+                    for (int q = 1; q < p; ++q)
+                        acc = O::cadd(acc, O::cmul(x[q], O::load(grow[q1 * (p - 1) + q - 1])));
+                    v[j][q1] = acc;
+                }
             } else if constexpr (A::fixed) {
                 C w[p > 1 ? p - 1 : 1] = {};
                 if constexpr (!ONE) {

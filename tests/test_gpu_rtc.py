@@ -1,9 +1,10 @@
 """GPU tests of runtime-compiled row kernels (kfa_rtc.hip) for lengths outside the
 built-in registry, and of the persisted table of tuned choices.
 
-Lengths whose radices are 2, 3, 4, 5 (float: also 7, 11, 13) and whose line fits one LDS
-line get the compile-time Stockham kernel k_fast compiled on first use (fixed point with
-the reference schedule of kf_factor, kiss_fft.c:306-328).  Results: bit-exact against the oracle for Q15 / Q31,
+Lengths whose radices are 2, 3, 4, 5, 7, 11, 13 and whose line fits one LDS line get the
+compile-time Stockham kernel k_fast compiled on first use (fixed point with the reference
+schedule of kf_factor, kiss_fft.c:306-328, and kf_bfly_generic's arithmetic for 7, 11
+and 13, kiss_fft.c:191-233).  Results: bit-exact against the oracle for Q15 / Q31,
 within the float bar of FP64 for float; the engine's own report (KISSFFT_AMD_VERBOSE=1)
 shows that the runtime-compiled variant ran.
 """
@@ -45,7 +46,8 @@ def rnd(rng, dtype, shape):
                                      ("float", 7680), ("float", 2401), ("float", 1001), ("float", 1331),
                                      ("float", 2197), ("float", 7 * 11 * 13 * 5), ("float", 784),
                                      ("int16_t", 1536), ("int16_t", 3000), ("int16_t", 6144),
-                                     ("int32_t", 1536), ("int32_t", 2187)])
+                                     ("int16_t", 1001), ("int16_t", 2401), ("int16_t", 1331),
+                                     ("int32_t", 1536), ("int32_t", 2187), ("int32_t", 1001), ("int32_t", 2401)])
 def test_rtc_c2c(rtc_env, capfd, dtype, n):
     lib = kissfft_amd.load(dtype)
     lib.L.kiss_fft_cleanup()
@@ -70,7 +72,7 @@ def test_rtc_c2c(rtc_env, capfd, dtype, n):
 
 
 @pytest.mark.parametrize("dtype,n", [("float", 3072 * 2), ("float", 6000), ("float", 2 * 2401), ("float", 2002),
-                                     ("int16_t", 3000)])
+                                     ("int16_t", 3000), ("int16_t", 2002), ("int32_t", 2 * 2401)])
 def test_rtc_real(rtc_env, capfd, dtype, n):
     lib = kissfft_amd.load(dtype)
     lib.L.kiss_fft_cleanup()
