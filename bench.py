@@ -258,6 +258,21 @@ def other_configs(lib, stream, dev) -> dict:
     rec("c2c_float_3000_x65536_runtime_compiled", t, 65536 * 16 * 3000, 65536, first_call_ms=round(first * 1e3, 1))
     del xr, yr
 
+    # a length of a generic radix (2401 = 7^4, kf_bfly_generic in the reference): the odd
+    # radix-7 DFT compiled at run time the same way
+    with torch.cuda.stream(stream):
+        xg = torch.rand(55900, 2401, 2, device=dev) * 2 - 1
+        yg = torch.empty_like(xg)
+    stream.synchronize()
+    cg = lib.fft_alloc(2401)
+    t0 = time.perf_counter()
+    lib.fft_batch(cg, xg, yg, 55900, 2401, 2401, stream=stream)
+    stream.synchronize()
+    first = time.perf_counter() - t0
+    t = _time(lambda: lib.fft_batch(cg, xg, yg, 55900, 2401, 2401, stream=stream), stream, 10)
+    rec("c2c_float_2401_x55900_runtime_compiled", t, 55900 * 16 * 2401, 55900, first_call_ms=round(first * 1e3, 1))
+    del xg, yg
+
     # 3-D complex float 64x96x120, 16 transforms
     dims = (64, 96, 120)
     with torch.cuda.stream(stream):
