@@ -563,8 +563,26 @@ struct FastStage {
             const int b = bidx(t, j);
             const int u = b / P;
             const St *row = stw + TWO + u * PR;
-            if constexpr (IsPacked<A>::value) {
-                static_assert(!RS::gen(S), "packed Q15 lines take radices 2, 3, 4, 5 only");
+            if constexpr (IsPacked<A>::value && RS::gen(S)) {
+                // kf_bfly_generic (kiss_fft.c:191-233) on packed words: C_FIXDIV, then per
+                // output the packed C_MULs with the combined entries (word pairs) summed by
+                // packed 16-bit adds (C_ADDTO into kiss_fft_scalar)
+                const uint2 *grow = reinterpret_cast<const uint2 *>(stw) + RS::gen_off(S) + u * RS::GR(S);
+                unsigned x[p];
+#pragma unroll
+                for (int q = 0; q < p; ++q)
+                    x[q] = q15p::fixdiv<p>(v[j][q].w);
+#pragma unroll
+                for (int q1 = 0; q1 < p; ++q1) {
+                    unsigned acc = x[0];
+#pragma unroll
+                    for (int q = 1; q < p; ++q) {
+                        const uint2 w = grow[q1 * (p - 1) + q - 1];
+                        acc = q15p::add(acc, q15p::cmul(x[q], w.x, w.y));
+                    }
+                    v[j][q1].w = acc;
+                }
+            } else if constexpr (IsPacked<A>::value) {
                 // stage table of pre-arranged twiddle word pairs (see kfa_q15p.hpp)
                 const uint2 *tab = reinterpret_cast<const uint2 *>(stw);
                 const uint2 *trow = tab + TWO + u * PR;
