@@ -44,7 +44,8 @@ ENGINE_SOURCES = ["kfa_engine.hip", "kfa_fast_reg.hip", "kfa_tile_reg.hip", "kfa
                   "kfa_rtc.hip"]
 # headers the runtime compiler needs for k_fast and the fused multi-batch launch
 # (kfa_rtc.hip), embedded in the library
-RTC_HEADERS = ["kfa_fast.hpp", "kfa_arith.hpp", "kfa_lines.hpp", "kfa_q15p.hpp", "kfa_tile.hpp", "kfa_blue.hpp"]
+RTC_HEADERS = ["kfa_fast.hpp", "kfa_arith.hpp", "kfa_lines.hpp", "kfa_q15p.hpp", "kfa_tile.hpp", "kfa_blue.hpp",
+               "kfa_q15w.hpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CC = os.environ.get("CC", "gcc")

@@ -26,6 +26,7 @@ struct integer_sequence {
 };
 template <class T, T N>
 using make_integer_sequence = __make_integer_seq<integer_sequence, T, N>;
+using __hip_internal::integral_constant;
 }  // namespace std
 #endif
 

@@ -166,6 +166,8 @@ static void add_xpose(std::vector<FastVariant> &v, const char *name)
         f.tw_fmt = TwPad<Q15P>::fmt;
         f.tw_len = RS::tw_len();
         f.name = name;
+        f.arith = ArithName<Q15P>::value;
+        f.xpose = true;
         f.launch = inv ? &launch_xpose<true, LPB, RS, WPE> : &launch_xpose<false, LPB, RS, WPE>;
         v.push_back(f);
     }

@@ -30,6 +30,7 @@ struct FastVariant {
     const char *arith = nullptr;
     bool db = false;
     bool fuse = false;
+    bool xpose = false;  // k_q15x (kfa_q15w.hpp: k_q15x_body<INV, LPB, Radices<tw_pad, radices...>>)
 };
 
 const std::vector<FastVariant> &fast_registry();

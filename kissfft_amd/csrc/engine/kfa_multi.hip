@@ -135,6 +135,16 @@ static bool reshape(const FastVariant &v, int NT, FastVariant &out, size_t &lds)
 {
     if (!v.arith || v.fuse || NT <= 0)
         return false;
+    if (v.xpose) {  // k_q15x: 16 lanes per line; lines per workgroup re-dealt (static LDS)
+        if (NT % 16 || NT / 16 > 32)
+            return false;
+        out = v;
+        out.LPB = NT / 16;
+        out.launch = nullptr;
+        out.rtc = nullptr;
+        lds = 0;
+        return true;
+    }
     const int pmin = *std::min_element(v.radices.begin(), v.radices.end());
     const double want = std::log((double)v.n / v.T);
     bool found = false;
@@ -165,7 +175,8 @@ static bool reshape(const FastVariant &v, int NT, FastVariant &out, size_t &lds)
 // fused launch: a kernel compiled at run time from those groups' own bodies (rtc_fused),
 // one grid whose workgroup ranges run the groups in order of size, largest first, so the
 // small groups fill the large ones' tail instead of each draining the device on its own.
-// Row-kernel groups of another size join reshaped to that size (reshape above); the size
+// Row-kernel groups (k_fast, and the Q15 wavefront-transpose kernel k_q15x) of another
+// size join reshaped to that size (reshape above); the size
 // is the one whose members carry the most work.  Groups that cannot join (Bluestein of
 // another size, fused real split / merge variants of another size, more than 64 KiB of
 // LDS), and every group when the runtime compiler is unavailable, keep their own launches.
@@ -187,8 +198,9 @@ static void fuse_groups(Multi &M)
     for (size_t i = 0; i < M.groups.size(); ++i) {
         const FastVariant *v = M.groups[i].v;
         if (!v->arith)
-            continue;  // another kernel than k_fast
-        const size_t lds = (v->db ? 2 : 1) * (size_t)v->LPB * v->n * (16 / v->tw_pad);
+            continue;  // another kernel than k_fast / k_q15x
+        // (k_q15x: its transpose buffer is static LDS)
+        const size_t lds = v->xpose ? 0 : (v->db ? 2 : 1) * (size_t)v->LPB * v->n * (16 / v->tw_pad);
         if (lds > kMaxLds)
             continue;
         const long long blocks = (M.groups[i].a.nlines + v->LPB - 1) / v->LPB;

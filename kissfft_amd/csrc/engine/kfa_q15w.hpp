@@ -23,9 +23,11 @@
 // positions of the values move.
 #pragma once
 
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 
 #include <utility>
+#endif
 
 #include "kfa_fast.hpp"
 #include "kfa_q15p.hpp"
@@ -505,10 +507,10 @@ __device__ __forceinline__ void xstages(std::integer_sequence<int, S...>, u32 (&
 }  // namespace q15w
 
 // Q15 c2c lines of the power-of-two schedule RS held by 16 lanes each, LPB lines per
-// workgroup, with the one LDS transpose of XposePlan in place of lane swaps.  WPE: the
-// wavefronts per SIMD the register allocation must leave room for.
-template <bool INV, int LPB, class RS, int WPE>
-__global__ void __launch_bounds__(16 * LPB, WPE) k_q15x(FastArgs a)
+// workgroup, with the one LDS transpose of XposePlan in place of lane swaps: the body for
+// workgroup `blk` of its launch (k_q15x below, or a range of a fused launch).
+template <bool INV, int LPB, class RS>
+__device__ __forceinline__ void k_q15x_body(const FastArgs &a, long long blk)
 {
     constexpr int T = 16;
     typedef XposePlan<RS, T> PL;
@@ -517,7 +519,7 @@ __global__ void __launch_bounds__(16 * LPB, WPE) k_q15x(FastArgs a)
     __shared__ __align__(16) u32 lds[LPB * q15w::kXposeStride];
     const int line = threadIdx.x / T;
     const int lane = threadIdx.x - line * T;
-    const long long gl = (long long)blockIdx.x * LPB + line;
+    const long long gl = blk * LPB + line;
     const bool valid = gl < a.nlines;
     const q15x2 *gin;
     q15x2 *gout;
@@ -550,6 +552,13 @@ __global__ void __launch_bounds__(16 * LPB, WPE) k_q15x(FastArgs a)
             hbm_st(gout + (long long)(y_lane + q15w::reg_field<PL, RS::K>(x, 0, PL::NBITS)) * oe,
                    __builtin_bit_cast(q15x2, v[x]));
     }
+}
+
+// WPE: the wavefronts per SIMD the register allocation must leave room for.
+template <bool INV, int LPB, class RS, int WPE>
+__global__ void __launch_bounds__(16 * LPB, WPE) k_q15x(FastArgs a)
+{
+    k_q15x_body<INV, LPB, RS>(a, blockIdx.x);
 }
 
 // Q15 c2c lines of the power-of-two schedule RS, T lanes per line, LPB lines per workgroup.
