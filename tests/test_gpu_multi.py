@@ -78,10 +78,10 @@ def test_float_mix(layout):
         assert info["grid"] > 0 and info["tiles"] > 0
     else:
         # compiled kernels for 256, 1200 inverse, r2c 4096, c2r 4000 and (compiled at run
-        # time) 8192 and 1001 = 7 * 11 * 13; Bluestein for 1009; 60, r2c 6, 1 and c2r 2 in
-        # the persistent launch
-        assert info["group_items"] == 6 and info["groups"] == 6 and info["bluestein_items"] == 1
-        assert info["persistent_items"] == 4
+        # time) 8192, 1001 = 7 * 11 * 13 and 60; Bluestein for 1009; r2c 6, 1 and c2r 2 (one
+        # stage each) in the persistent launch
+        assert info["group_items"] == 7 and info["groups"] == 7 and info["bluestein_items"] == 1
+        assert info["persistent_items"] == 3
     m.exec()
     for (k, n, b), (x, y) in zip(FLOAT_ITEMS, bufs):
         say("item", k, n, b)
