@@ -101,14 +101,18 @@ def cpu_baseline(threads: int) -> dict | None:
 
 
 def _time(fn, stream, reps: int) -> float:
-    """Seconds per call of `reps` back-to-back calls after one untimed call; the wall time of
-    that first call (plans, tables, first-use timing of kernel variants when the process has
-    not run the workload before) is kept in _time.first_s."""
+    """Seconds per call of `reps` back-to-back calls after two untimed calls; the wall time
+    of the first call (plans, tables, first-use timing of kernel variants when the process
+    has not run the workload before) is kept in _time.first_s.  The second untimed call
+    runs the workload once more as it will run from then on (the first calls of a workload
+    that takes a temporary map its pool memory)."""
     import torch
     t0 = time.perf_counter()
     fn()
     stream.synchronize()
     _time.first_s = time.perf_counter() - t0
+    fn()
+    stream.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record(stream)
