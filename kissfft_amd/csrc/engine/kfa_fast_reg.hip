@@ -262,7 +262,11 @@ const std::vector<FastVariant> &fast_registry()
         add_variant<F32, FK_C2C, 128, 2, false, false, 5, 5, 5, 4>(*v, KFA_F32, "c2c500_5554_t128x2_sb");
         add_variant<F32, FK_C2C, 125, 2, false, false, 5, 5, 5, 4>(*v, KFA_F32, "c2c500_5554_t125x2_sb");
         add_variant<F32, FK_C2C, 192, 1, false, false, 3, 16, 8, 8>(*v, KFA_F32, "c2c3072_31688_t192_sb");
+        // (65536 lines: 8888_t256 5.11-5.14 TB/s, 8888_t512 4.92, 161616_t256 4.82-4.88,
+        // 161616_t256 double-buffered 4.59: not registered)
+        add_variant<F32, FK_C2C, 256, 1, false, false, 8, 8, 8, 8>(*v, KFA_F32, "c2c4096_8888_t256_sb");
         add_variant<F32, FK_C2C, 256, 1, false, false, 16, 16, 16>(*v, KFA_F32, "c2c4096_161616_t256_sb");
+        add_variant<F32, FK_C2C, 512, 1, false, false, 8, 8, 8, 8>(*v, KFA_F32, "c2c4096_8888_t512_sb");
         // ---- Q15 (reference schedules), wavefront-resident lines: one LDS transpose ----
         add_xpose<8, 1, 2, 4, 4, 4, 4>(*v, "q15x_c2c512_t16x8");
         add_xpose<16, 1, 2, 4, 4, 4, 4>(*v, "q15x_c2c512_t16x16");
