@@ -295,8 +295,9 @@ const std::vector<FastVariant> &fast_registry()
         add_fused<Q15, FK_R2C, 256, 1, 2, 4, 4, 4, 4, 4>(*v, KFA_Q15, "q15r2c2048_t256_fused");
         add_fused<Q15, FK_C2R, 512, 1, 2, 4, 4, 4, 4, 4>(*v, KFA_Q15, "q15c2r2048_t512_fused");
         // ---- Q31 (reference schedules; int64 products, 8 bytes per complex) ----
-        add_variant<Q31, FK_C2C, 128, 2, false, false, 2, 4, 4, 4, 4>(*v, KFA_Q31, "q31c2c512_t128x2_sb");
+        // (2^19 lines: t64x4 875-880 us, t128x2 949-1003 us)
         add_variant<Q31, FK_C2C, 64, 4, false, false, 2, 4, 4, 4, 4>(*v, KFA_Q31, "q31c2c512_t64x4_sb");
+        add_variant<Q31, FK_C2C, 128, 2, false, false, 2, 4, 4, 4, 4>(*v, KFA_Q31, "q31c2c512_t128x2_sb");
         add_variant<Q31, FK_C2C, 256, 1, false, false, 5, 5, 3, 4, 4>(*v, KFA_Q31, "q31c2c1200_t256_sb");
         add_real<Q31, 256, 1, false, 2, 4, 4, 4, 4, 4>(*v, KFA_Q31, "q31real2048_t256_sb");
         add_real<Q31, 256, 1, false, 5, 5, 5, 4, 4>(*v, KFA_Q31, "q31real2000_t256_sb");
