@@ -45,7 +45,7 @@ def rnd(rng, dtype, shape):
 # (lengths no earlier test file compiles: the test checks that this test's compile
 # reached the disk cache, and a kernel another test loaded is not compiled again; and
 # lengths whose kernels fit the registers: Q31 3003 = 13*11*7*3 spills at every thread
-# shape and stays on the general kernel)
+# shape, the widest included, and stays on the general kernel)
 @pytest.mark.parametrize("dtype,n", [("float", 1536), ("float", 2187), ("float", 3000), ("float", 6561),
                                      ("float", 7680), ("float", 2401), ("float", 3003), ("float", 1331),
                                      ("float", 2197), ("float", 7 * 11 * 13 * 5), ("float", 784),
@@ -69,6 +69,17 @@ def test_rtc_c2c(rtc_env, capfd, dtype, n):
             assert err <= ABS_FLOAT_BOUND
         else:
             np.testing.assert_array_equal(y, Oracle(dtype).fft(x, inv))
+    # every other input element (in_stride 2) and output rows n + 3 apart: the kernel's
+    # general-stride body (fixed point otherwise takes the unit-stride one)
+    xs = rnd(rng, dtype, (5, 2 * n, 2))
+    ys = np.zeros((5, n + 3, 2), lib.scalar)
+    lib.fft_batch(lib.fft_alloc(n), xs, ys, 5, 2 * n, n + 3, in_stride=2)
+    xd = np.ascontiguousarray(xs[:, ::2])
+    if dtype == "float":
+        assert rel_l2(cplx(ys[:, :n]), np.fft.fft(cplx(xd), axis=-1)) <= ABS_FLOAT_BOUND
+    else:
+        np.testing.assert_array_equal(ys[:, :n], Oracle(dtype).fft(xd))
+    assert (ys[:, n:] == 0).all()
     out = capfd.readouterr().err
     assert f"n={n} kind=0 inverse=0: fast variant rtc_c2c{n}" in out, out[-2000:]
     assert f"n={n} kind=0 inverse=1: fast variant rtc_c2c{n}" in out, out[-2000:]
