@@ -60,11 +60,13 @@ typedef struct {
 /* Builds the plans and the launch table.  NULL on failure (see kiss_fft_last_error). */
 kiss_fft_multi KISS_FFT_API kiss_fft_multi_alloc(const kiss_fft_multi_item *items, int nitems);
 
-/* Runs every item: one persistent launch on `stream` (a hipStream_t; NULL: the calling
- * thread's stream, synchronised before returning). 0 or a negative code. */
+/* Runs every item: the multi-batch's launches (the fused launch of its compiled-kernel
+ * groups, the persistent launch of the other single-pass items, multi-pass launches) back
+ * to back on `stream` (a hipStream_t; NULL: the calling thread's stream, synchronised
+ * before returning). 0 or a negative code. */
 int KISS_FFT_API kiss_fft_multi_exec(kiss_fft_multi m, void *stream);
 
-/* Captures the launch on `stream` (not NULL) into a graph and instantiates it. */
+/* Captures the launches on `stream` (not NULL) into a graph and instantiates it. */
 int KISS_FFT_API kiss_fft_multi_capture(kiss_fft_multi m, void *stream);
 
 /* Launches the captured graph on `stream` (NULL: the calling thread's stream, synchronised). */
