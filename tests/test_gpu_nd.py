@@ -80,6 +80,39 @@ def test_q15_nd_bit_exact_vs_oracle_larger_shapes():
         np.testing.assert_array_equal(ndri(lib, dims, X), O.fftndri(X, dims))
 
 
+@pytest.mark.parametrize("dtype", ["int16_t", "int32_t", "float"])
+def test_nd_generic_radix_axes(dtype):
+    """N-D arrays whose axes have the generic radices 7, 11, 13: a last axis of 1001 (its
+    contiguous lines on the kernel compiled at run time), an outer axis of 143 and a real
+    last axis of 2002, against the oracle (fixed point, bit-exact) or FP64 (float)."""
+    lib = kissfft_amd.load(dtype)
+    rng = np.random.default_rng(44)
+    lo, hi, dt = {"int16_t": (-32768, 32768, np.int16), "int32_t": (-(1 << 31), 1 << 31, np.int32),
+                  "float": (None, None, np.float32)}[dtype]
+    def rnd(shape):
+        if dtype == "float":
+            return rng.uniform(-1, 1, shape).astype(np.float32)
+        return rng.integers(lo, hi, shape, dtype=np.int64).astype(dt)
+    O = Oracle(dtype) if dtype != "float" else None
+    for dims in [(6, 1001), (143, 12), (3, 77, 13)]:
+        x = rnd((2,) + dims + (2,))
+        say("case nd generic", dtype, dims)
+        got = nd(lib, dims, x)
+        if O is not None:
+            np.testing.assert_array_equal(got, O.fftnd(x, dims))
+        else:
+            axes = tuple(range(1, len(dims) + 1))
+            assert rel_l2(cplx(got), np.fft.fftn(cplx(x), axes=axes)) <= ABS_FLOAT_BOUND, dims
+    dims = (5, 2002)
+    x = rnd((2,) + dims)
+    say("case ndr generic", dtype, dims)
+    got = ndr(lib, dims, x)
+    if O is not None:
+        np.testing.assert_array_equal(got, O.fftndr(x, dims))
+    else:
+        assert rel_l2(cplx(got), np.fft.rfftn(x.astype(np.float64), axes=(1, 2))) <= ABS_FLOAT_BOUND
+
+
 def test_float_nd_workload(golden_float):
     # 16 transforms of 64x96x120, re/im uniform on [-1,1), seed 4
     lib = kissfft_amd.load("float")
