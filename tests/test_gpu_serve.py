@@ -52,7 +52,7 @@ def test_mixed_stream_equals_alone_float():
     srv = lib.server(0, 4, 64)
     rng = np.random.default_rng(61)
     reqs = []
-    sizes = [256, 1000, 1009, 3072, 1200, 4096, 4000, 65536]
+    sizes = [256, 1000, 1009, 3072, 1200, 4096, 4000, 65536, 1001, 2002]  # (1001 = 7*11*13)
     for i in range(60):
         n = int(rng.choice(sizes))
         kind = int(rng.integers(0, 4))
@@ -113,7 +113,7 @@ def test_q15_served_bit_exact():
     O = Oracle("int16_t")
     rng = np.random.default_rng(63)
     jobs = []
-    for n in (512, 1200, 512, 60, 512, 4096):
+    for n in (512, 1200, 512, 60, 512, 4096, 1001, 2401):
         x = rng.integers(-32768, 32768, (int(rng.integers(1, 40)), n, 2)).astype(np.int16)
         y = np.zeros_like(x)
         jobs.append((n, x, y, srv.submit(C2C_FWD, n, x, y, x.shape[0])))
