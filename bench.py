@@ -423,10 +423,15 @@ def serving(lib, dev, count: int, rate: float, streams: int = 0) -> dict:
     transforms = sum(r["batch"] for r in reqs)
 
     srv = lib.server(streams=streams)
-    # warm-up: plans, twiddles and variant choice of every (kind, n), one transform each
+    # warm-up: plans, twiddles and variant choice of every (kind, n), one transform each;
+    # each first request is timed (submit to completion): the cold-start latency a new
+    # server pays once per (kind, n)
+    cold = {}
     for key in inputs:
         warm_out = torch.empty(W.request_bytes(key[0], key[1], 1)[1] // 4, device=dev)
+        tw = time.perf_counter()
         srv.wait(srv.submit(kinds[key[0]], key[1], ibufs[key], warm_out, 1))
+        cold[f"{key[0]}_{key[1]}"] = round((time.perf_counter() - tw) * 1e3, 1)
     srv.reset_stats()
 
     def run(open_loop: bool, stretch: float = 1.0):
@@ -514,6 +519,7 @@ def serving(lib, dev, count: int, rate: float, streams: int = 0) -> dict:
                      "open_loop_p50_ms": ms(q(fifo, 0.50)), "open_loop_p99_ms": ms(q(fifo, 0.99)),
                      "open_loop_rate_per_s": rate},
         "speedup_vs_per_call": round(sat_gbps / pc_gbps, 2),
+        "cold_first_request_ms": {"per_kind_n": cold, "max": max(cold.values()), "sum": round(sum(cold.values()), 1)},
         "served_equals_per_call": bool(same),
     }
 
