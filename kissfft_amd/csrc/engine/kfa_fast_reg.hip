@@ -233,6 +233,12 @@ const std::vector<FastVariant> &fast_registry()
         add_real<F32, 125, 2, false, 4, 5, 5, 5>(*v, KFA_F32, "real500_4555_t125x2_sb");
         add_fused<F32, FK_R2C, 50, 4, 4, 5, 5, 5>(*v, KFA_F32, "r2c500_4555_t50x4_fused");
         add_fused<F32, FK_C2R, 50, 4, 5, 5, 5, 4>(*v, KFA_F32, "c2r500_5554_t50x4_fused");
+        // N = 256 ... 2048 (halves 128 ... 1024), the shapes of the complex variants of the
+        // half lengths: common real sizes start without a runtime compile
+        add_real<F32, 16, 16, false, 8, 16>(*v, KFA_F32, "real128_816_t16x16_sb");
+        add_real<F32, 16, 16, false, 16, 16>(*v, KFA_F32, "real256_1616_t16x16_sb");
+        add_real<F32, 64, 4, false, 8, 8, 8>(*v, KFA_F32, "real512_888_t64x4_sb");
+        add_real<F32, 64, 2, false, 16, 16, 4>(*v, KFA_F32, "real1024_16164_t64x2_sb");
         // ---- float complex ----
         add_variant<F32, FK_C2C, 240, 1, true, false, 4, 4, 3, 5, 5>(*v, KFA_F32, "c2c1200_44355_t240_db");
         add_variant<F32, FK_C2C, 240, 1, false, false, 4, 4, 3, 5, 5>(*v, KFA_F32, "c2c1200_44355_t240_sb");
