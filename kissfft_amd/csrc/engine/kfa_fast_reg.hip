@@ -239,6 +239,9 @@ const std::vector<FastVariant> &fast_registry()
         add_real<F32, 16, 16, false, 16, 16>(*v, KFA_F32, "real256_1616_t16x16_sb");
         add_real<F32, 64, 4, false, 8, 8, 8>(*v, KFA_F32, "real512_888_t64x4_sb");
         add_real<F32, 64, 2, false, 16, 16, 4>(*v, KFA_F32, "real1024_16164_t64x2_sb");
+        // N = 3072 (half 1536), a size of the serving mix
+        add_real<F32, 192, 1, false, 8, 8, 8, 3>(*v, KFA_F32, "real1536_8883_t192_sb");
+        add_real<F32, 96, 2, false, 16, 16, 3, 2>(*v, KFA_F32, "real1536_161632_t96x2_sb");
         // ---- float complex ----
         add_variant<F32, FK_C2C, 240, 1, true, false, 4, 4, 3, 5, 5>(*v, KFA_F32, "c2c1200_44355_t240_db");
         add_variant<F32, FK_C2C, 240, 1, false, false, 4, 4, 3, 5, 5>(*v, KFA_F32, "c2c1200_44355_t240_sb");
